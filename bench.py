@@ -1,0 +1,290 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-process advection path on one MI355X.
+
+Workload (one rank's share of a 512^3 ABC domain split 4x4x4): the own
+block of rank 21 and its six face neighbours, 7 x 129^3 ghosted blocks
+resident in HBM; 262,144 seeds on a stride-2 lattice over the own block;
+runge_kutta_4 with the reference's frozen-derivative stepper, dt = 0.005,
+up to 1000 steps (remaining_iterations = 1002); particles continue across
+the 7 blocks and stop on leaving their union or when |u| < 1e-6.
+
+One step = one pass of the hot path over that batch: seed lattice ->
+union trace -> prune -> index -> colour, all on the device with the inputs
+(fields) already resident.  The value is integrator steps (particle-steps)
+per second of that whole pass, FP32 field.  BF16 / FP16 field storage and
+the consistent-weight interpolation are reported beside it, with their
+endpoint error against the FP64 CPU oracle.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK4 particle-steps/s on 1 MI355X (FP32 vs BF16 field) + % HBM peak; p99 latency"
+HBM_PEAK_GBS = 8000.0
+R = 1002          # remaining_iterations: 1000 integrator steps
+DT = 0.005
+EXTRA_SLOTS = 30  # per-seed region beyond R for union segments
+GATHER_BYTES = {0: 96, 1: 48, 2: 48}  # 8 corners x 3 components x storage bytes
+STORE_BYTES = 16                      # one vec4 FP32 vertex per step
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Pipeline:
+    """Device buffers and calls of one step, reused across steps."""
+
+    def __init__(self, pa, ctx, fields, share, storage, interp, n_max):
+        self.pa, self.ctx, self.lib = pa, ctx, pa.lib()
+        self.share = share
+        t = pa.particle_tracer(ctx)
+        t.set_local_vector_field(fields[0])
+        t.set_neighbor_vector_fields(fields[1:], share.neighbor_ranks)
+        t.set_neighbor_face_masks(share.face_masks)
+        t.set_integrator(pa.RUNGE_KUTTA_4)
+        t.set_step_size(DT)
+        t.set_stop_rule(pa.STOP_NORM, 1e-6)
+        t.set_interpolation(interp)
+        self.tracer = t
+        self.region = R + EXTRA_SLOTS
+        self.seeds = ctx.alloc(n_max * 48)
+        self.ub = t.union_buffers(n_max, self.region, endpoints=True)
+        self.cap = n_max * self.region
+        self.pruned = ctx.alloc(self.cap * 16)
+        self.offs = ctx.alloc((n_max + 1) * 8)
+        self.indices = ctx.alloc(self.cap * 4)
+        self.colors = ctx.alloc(self.cap * 16)
+        loc = share.local
+        self.seed_args = (pa._f3(loc.offset), pa._f3(loc.size), pa._f3(loc.spacing * np.float32(2)))
+
+    def step(self):
+        pa, lib, ctx = self.pa, self.lib, self.ctx
+        n = C.c_size_t()
+        o, s, st = self.seed_args
+        ctx.check(lib.pa_gpu_seed_generate(ctx.handle, o, s, st, C.c_int32(R), C.c_int32(21),
+                                           C.c_void_p(self.seeds.ptr), C.c_size_t(self.seeds.nbytes // 48),
+                                           C.byref(n)), "seeds")
+        parts = pa.Particles(self.seeds, n.value)
+        r = self.tracer.trace_union(parts, self.region, buffers=self.ub)
+        if r["continuation"].n:
+            raise RuntimeError("segment region overflow: %d seeds need a continuation pass" % r["continuation"].n)
+        m = C.c_size_t()
+        ctx.check(lib.pa_gpu_prune_counted(ctx.handle, C.c_void_p(self.ub["vertices"].ptr), C.c_size_t(self.region),
+                                           C.c_void_p(self.ub["counts"].ptr), n, C.c_void_p(self.pruned.ptr),
+                                           C.c_size_t(self.cap), C.byref(m), C.c_void_p(self.offs.ptr)), "prune")
+        ni = C.c_size_t()
+        ctx.check(lib.pa_gpu_index_generate(ctx.handle, C.c_void_p(self.pruned.ptr), m, C.c_void_p(self.indices.ptr),
+                                            C.byref(ni)), "index")
+        ctx.check(lib.pa_gpu_color_generate(ctx.handle, C.c_void_p(self.pruned.ptr), m, C.c_void_p(self.indices.ptr),
+                                            ni, C.c_int(2), C.c_float(0.75), C.c_void_p(self.colors.ptr)), "color")
+        return dict(particles=n.value, steps=r["total_steps"], vertices=m.value, indices=ni.value)
+
+
+def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup):
+    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds)
+    for _ in range(warmup):
+        info = p.step()
+    ctx.synchronize()
+    ctx.timing_reset()
+    ctx.timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    total = 0
+    for _ in range(steps):
+        info = p.step()
+        total += info["steps"]
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.timing(False)
+    kt = {k: ctx.kernel_time(k) for k in ("seeds", "trace_union", "prune_scan", "prune_counted", "index", "color")}
+    trace_ms = kt["trace_union"][0] / max(kt["trace_union"][1], 1)
+    return p, dict(wall_s=wall, ms_per_step=1e3 * wall / steps, steps_per_pass=total // steps,
+                   value=total / wall, trace_ms=trace_ms, info=info,
+                   stage_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()})
+
+
+def endpoint_errors(pa, orc, p, share, storage, interp, sample_stride):
+    """Endpoints of a sample of seeds against the FP64 oracle (consistent
+    interpolant, same stepper, same union walk and stop rule)."""
+    from pars_amd import workloads as W
+
+    ctx = p.ctx
+    p.step()
+    n = p.seeds.nbytes // 48
+    seeds = p.seeds.download(pa.PARTICLE_DTYPE, n)
+    ends = p.ub["endpoints"].download(np.float32, 4 * n).reshape(n, 4)
+    ssteps = p.ub["seed_steps"].download(np.int32, n)
+    sel = np.arange(0, n, sample_stride)
+    blocks = [share.local] + list(share.neighbors)
+    f64 = [orc.Field64(W.abc_block_f64(b, share.h), b.offset.astype(np.float64), b.size.astype(np.float64),
+                       [share.h] * 3) for b in blocks]
+    e64, s64 = orc.endpoints_f64(seeds["position"][sel].astype(np.float64), R, f64[0], neighbors=f64[1:],
+                                 neighbor_rank=share.neighbor_ranks, face_masks=share.face_masks, dt=DT,
+                                 integrator=orc.RK4, union=True, stop_rule=orc.STOP_NORM, stop_eps=1e-6)
+    same = ssteps[sel] == s64
+    d = np.linalg.norm(ends[sel][same, :3].astype(np.float64) - e64[same, :3], axis=1)
+    extent = float(share.local.size[0])
+    return dict(sample=int(len(sel)), same_step_count_frac=float(same.mean()),
+                max_endpoint_err=float(d.max()) if len(d) else None,
+                rms_endpoint_err=float(np.sqrt((d ** 2).mean())) if len(d) else None,
+                max_err_over_block_extent=float(d.max() / extent) if len(d) else None,
+                p99_endpoint_err=float(np.percentile(d, 99)) if len(d) else None)
+
+
+def cpu_baseline(orc, share, sample_stride, threads):
+    """The oracle's FP32 restatement of the same union trace on host cores."""
+    from pars_amd import workloads as W
+
+    blocks = [share.local] + list(share.neighbors)
+    f32 = [orc.Field(orc.abc_field_f64(b.ghosted, b.int_offset, share.h).astype(np.float32), b.offset, b.size,
+                     b.spacing) for b in blocks]
+    seeds = orc.seed_generate(share.local.offset, share.local.size, share.local.spacing * np.float32(2), R, 21)
+    sample = seeds[::sample_stride].copy()
+    orc.set_threads(threads)
+    t0 = time.perf_counter()
+    u = orc.trace_union(sample, f32[0], f32[1:], share.neighbor_ranks, share.face_masks, dt=DT, integrator=orc.RK4,
+                        stop_rule=orc.STOP_NORM, stop_eps=1e-6, vertex_capacity=len(sample) * (R + EXTRA_SLOTS))
+    dt = time.perf_counter() - t0
+    return dict(value=u["total_steps"] / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample="every %dth seed of the 262144 (%d seeds, %d steps), union trace with vertex output, "
+                       "%.2f s" % (sample_stride, len(sample), u["total_steps"], dt))
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--forms", default="fp32,bf16,fp16,fp32-consistent")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the per-rank share on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import pars_amd as pa
+    from pars_amd import workloads as W
+
+    ctx = pa.Context(0)
+    dev = ctx.device_info()
+    log("device:", dev)
+    share = W.share(512, 64, 21)
+    blocks = [share.local] + list(share.neighbors)
+    n_seeds = pa.seed_generator.count(share.local.size, share.local.spacing * np.float32(2))[0]
+    assert n_seeds == 262144
+
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    storage_of = {"fp32": pa.FP32, "bf16": pa.BF16, "fp16": pa.FP16, "fp32-consistent": pa.FP32}
+    interp_of = {"fp32-consistent": pa.INTERP_CONSISTENT}
+    fields = {}
+    results = {}
+    pipes = {}
+    for form in forms:
+        st = storage_of[form]
+        if st not in fields:
+            t0 = time.perf_counter()
+            fields[st] = [pa.VectorField.synthesize(ctx, "abc", W.ABC, b.ghosted, b.int_offset, share.h, b.offset,
+                                                    b.size, b.spacing, st) for b in blocks]
+            log("fields %s: %.1f MB in %.2f s" % (form, sum(f.nbytes() for f in fields[st]) / 1e6,
+                                                  time.perf_counter() - t0))
+        interp = interp_of.get(form, pa.INTERP_REFERENCE)
+        p, res = run_form(pa, ctx, fields[st], share, st, interp, n_seeds, args.steps, args.warmup)
+        log("%s: %.3f ms/step, %.3e steps/s, trace kernel %.3f ms, %s" %
+            (form, res["ms_per_step"], res["value"], res["trace_ms"], res["info"]))
+        results[form] = res
+        pipes[form] = (p, st, interp)
+
+    if args.calib_bytes:
+        a = ctx.alloc(args.calib_bytes)
+        b = ctx.alloc(args.calib_bytes)
+        for _ in range(3):
+            ctx.check(pa.lib().pa_gpu_copy(ctx.handle, C.c_void_p(b.ptr), C.c_void_p(a.ptr),
+                                           C.c_size_t(args.calib_bytes)), "copy")
+        ctx.synchronize()
+
+    base = results[forms[0]]
+    steps_per_pass = base["steps_per_pass"]
+    st0 = storage_of[forms[0]]
+    alg_bytes = steps_per_pass * (GATHER_BYTES[st0] + STORE_BYTES)
+    achieved = alg_bytes / (base["trace_ms"] * 1e-3) / 1e9
+    traffic = load_traffic("trace_union_config2_fp32.traffic.json")
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=(traffic or {}).get("bytes_per_launch"),
+                    kernel="trace_union (k_trace_union<FP32, RK4, reference>)",
+                    algorithmic_bytes_per_step=GATHER_BYTES[st0] + STORE_BYTES,
+                    steps_per_launch=steps_per_pass, avg_launch_ms=base["trace_ms"],
+                    effective_gather_gbs=steps_per_pass * GATHER_BYTES[st0] / (base["trace_ms"] * 1e-3) / 1e9)
+
+    extra = {}
+    for form in forms:
+        r = results[form]
+        st = storage_of[form]
+        extra[form] = dict(value=r["value"], ms_per_step=r["ms_per_step"], trace_kernel_ms=r["trace_ms"],
+                           trace_steps_per_s=r["steps_per_pass"] / (r["trace_ms"] * 1e-3),
+                           steps_per_pass=r["steps_per_pass"], stage_ms=r["stage_ms"],
+                           vs_fp32=r["value"] / base["value"],
+                           trace_vs_fp32=(r["steps_per_pass"] / r["trace_ms"]) /
+                                         (base["steps_per_pass"] / base["trace_ms"]),
+                           effective_gather_gbs=r["steps_per_pass"] * GATHER_BYTES[st] / (r["trace_ms"] * 1e-3) / 1e9)
+
+    orc = None
+    if not args.no_errors or not args.no_cpu_baseline:
+        from oracle import oracle as orc
+
+        orc.build()
+    if not args.no_errors:
+        orc.set_threads(args.cpu_threads)
+        for form in forms:
+            p, st, interp = pipes[form]
+            t0 = time.perf_counter()
+            extra[form]["fp64_error"] = endpoint_errors(pa, orc, p, share, st, interp, 32)
+            log("errors %s (%.1f s): %s" % (form, time.perf_counter() - t0, extra[form]["fp64_error"]))
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(orc, share, 2, args.cpu_threads)
+        log("cpu baseline:", cpu)
+
+    line = dict(
+        metric=METRIC, value=base["value"], unit="particle-steps/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=base["ms_per_step"], higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(workload="one rank's share: ABC 512^3 split 4x4x4, rank 21 + 6 face neighbours "
+                             "(7 x 129^3 ghosted FP32 vec4 blocks, 240 MB), 262144 seeds stride 2, "
+                             "runge_kutta_4 (reference frozen-derivative stepper), dt 0.005, <= 1000 steps, "
+                             "union-of-7 continuation, stop |u| < 1e-6; step = seeds + trace + prune + index + "
+                             "colour (hsv_constant_s 0.75)",
+                    seeds=n_seeds, remaining_iterations=R, dt=DT, field="ABC A=sqrt3 B=sqrt2 C=1",
+                    interpolation="reference" if forms[0] == "fp32" else forms[0], device=dev["name"]),
+        roofline=roofline, cpu_baseline=cpu, forms=extra,
+        bf16_vs_fp32=extra.get("bf16", {}).get("vs_fp32"),
+    )
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,330 @@
+/*
+ * pa_gpu.h — C ABI of the MI355X (gfx950) implementation of pars'
+ * per-process advection path.
+ *
+ * Every entry point stands in for one stage of the reference `pa` library
+ * (acdemiralp/pars); the comment above each names the reference interface it
+ * replaces (file:line).  The C++ classes in pars_amd/include/pa_amd/ wrap
+ * these functions with the reference's class and method names.
+ *
+ * Conventions
+ *   - Every function returns pa_status (0 = PA_OK).  pa_gpu_last_error(ctx)
+ *     returns the message of the last failure on that context.  No C++
+ *     exception crosses this boundary.
+ *   - Pointers named d_* are device (HBM) pointers, e.g. from pa_gpu_malloc;
+ *     pointers named h_* or without prefix are host pointers.
+ *   - Work is ordered on the context's HIP stream.  Calls that return host
+ *     values (counts, sizes) synchronise that stream before returning; the
+ *     others return once the work is enqueued.  pa_gpu_synchronize() waits.
+ *   - One host thread drives a context at a time (the reference runs its
+ *     stages under MPI_THREAD_SERIALIZED, pars/source/pipeline.cpp:14).
+ */
+#ifndef PA_GPU_H
+#define PA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PA_GPU_ABI_VERSION 1
+
+typedef enum pa_status {
+  PA_OK = 0,
+  PA_E_INVALID = 1,      /* invalid argument */
+  PA_E_UNSUPPORTED = 2,  /* valid in the reference but not implemented on the GPU (e.g. odeint dopri5) */
+  PA_E_DEVICE = 3,       /* HIP runtime error */
+  PA_E_NOMEM = 4,        /* device allocation failed */
+  PA_E_CAPACITY = 5,     /* a caller-provided buffer is too small */
+  PA_E_STATE = 6         /* call order violated (e.g. trace without a field) */
+} pa_status;
+
+typedef struct pa_gpu_ctx pa_gpu_ctx;
+typedef struct pa_gpu_field pa_gpu_field;
+typedef struct pa_gpu_tracer pa_gpu_tracer;
+
+/* pa::particle (pa/include/pa/math/particle.hpp:30-35).  48 bytes: the
+ * Eigen::Vector4f member gives the reference struct 16-byte alignment, so
+ * 40 bytes of members are padded to 48. */
+typedef struct pa_particle {
+  float   position[4];
+  int32_t remaining_iterations;
+  int32_t vector_field_index; /* -1 = own block, 0..5 = neighbour -x,+x,-y,+y,-z,+z */
+  int32_t original_rank;
+  int32_t original_voxel[3];
+  int32_t reserved_[2];
+} pa_particle;
+
+/* Field storage on the device: padded 4-component nodes. */
+typedef enum pa_storage { PA_STORAGE_FP32 = 0, PA_STORAGE_BF16 = 1, PA_STORAGE_FP16 = 2 } pa_storage;
+
+/* pa::variant_integrator alternatives (pa/include/pa/math/integrators.hpp:16-33),
+ * same order.  The reference's system() freezes the derivative at the last
+ * vertex (pa/source/stages/particle_tracer.cpp:315-318), which the GPU
+ * reproduces for EULER and RUNGE_KUTTA_4.  PA_INTEGRATOR_CLASSIC_RK4 is the
+ * textbook four-evaluation RK4, a build extension. */
+typedef enum pa_integrator {
+  PA_INTEGRATOR_EULER = 0,
+  PA_INTEGRATOR_MODIFIED_MIDPOINT = 1,
+  PA_INTEGRATOR_RUNGE_KUTTA_4 = 2,
+  PA_INTEGRATOR_RUNGE_KUTTA_CASH_KARP_54 = 3,
+  PA_INTEGRATOR_RUNGE_KUTTA_DORMAND_PRINCE_5 = 4,
+  PA_INTEGRATOR_RUNGE_KUTTA_FEHLBERG_78 = 5,
+  PA_INTEGRATOR_ADAMS_BASHFORTH_2 = 6,
+  PA_INTEGRATOR_ADAMS_BASHFORTH_MOULTON_2 = 7,
+  PA_INTEGRATOR_CLASSIC_RK4 = 100
+} pa_integrator;
+
+/* Stop test before each step: the reference's vector.isZero()
+ * (particle_tracer.cpp:312; every |v_i| <= 1e-5f) or the extension
+ * |v| < eps. */
+typedef enum pa_stop_rule { PA_STOP_ISZERO = 0, PA_STOP_NORM = 1 } pa_stop_rule;
+
+/* Interpolation weights.  PA_INTERP_REFERENCE reproduces
+ * vector_field::interpolate (pa/source/math/vector_field.cpp:22-48): cell
+ * from floor((p-o)/h), weight from fmod(p-o, h)/h, which reads one cell too
+ * far when the rounded quotient lands on an integer the true quotient is just
+ * below.  PA_INTERP_CONSISTENT (extension) uses w = q - floor(q) with the
+ * same cell: the intended trilinear interpolant, and three divisions and
+ * three remainders cheaper per evaluation. */
+typedef enum pa_interp { PA_INTERP_REFERENCE = 0, PA_INTERP_CONSISTENT = 1 } pa_interp;
+
+/* pa::color_generator::mode (pa/include/pa/stages/color_generator.hpp:12-19),
+ * same order; PA_COLOR_MAGNITUDE is a build extension (hue from segment
+ * length / parameter). */
+typedef enum pa_color_mode {
+  PA_COLOR_HSL_CONSTANT_S = 0,
+  PA_COLOR_HSL_CONSTANT_L = 1,
+  PA_COLOR_HSV_CONSTANT_S = 2,
+  PA_COLOR_HSV_CONSTANT_V = 3,
+  PA_COLOR_RGB = 4,
+  PA_COLOR_MAGNITUDE = 100
+} pa_color_mode;
+
+/* pa::ftle_map_generator::mode (pa/include/pa/stages/ftle_map_generator.hpp:16-20). */
+typedef enum pa_ftle_mode { PA_FTLE_REGULAR = 0, PA_FTLE_FRACTIONAL_ANISOTROPY = 1 } pa_ftle_mode;
+
+/* pa::particle_tracer::round_info without the two hand-off maps
+ * (pa/include/pa/stages/particle_tracer.hpp:26-37). */
+typedef struct pa_round_info {
+  uint64_t maximum_remaining_iterations;
+  uint64_t vertices_per_integral_curve;
+  uint64_t vertices_per_integral_curve_last;
+  uint64_t integral_curve_offset;
+  uint64_t integral_curve_count;
+} pa_round_info;
+
+/* The two hand-off maps of round_info (particle_tracer.hpp:34-35), compacted
+ * on the device into one caller buffer: first out_of_bounds_particles for
+ * neighbour slots 0..5, then neighbor_out_of_bounds_particles for slots
+ * 0..5, each group in source-particle order.  counts[kind][slot] is set on
+ * return.  The reference keys the maps by neighbour rank; the slot is the
+ * index into the neighbour array. */
+typedef struct pa_handoff {
+  pa_particle* d_particles; /* device buffer, capacity >= particle count */
+  int64_t*     d_sources;   /* optional: source particle index per entry */
+  size_t       capacity;
+  size_t       counts[2][6];
+} pa_handoff;
+
+/* ------------------------------------------------------------------ context */
+pa_status   pa_gpu_ctx_create(int device, pa_gpu_ctx** out);
+pa_status   pa_gpu_ctx_destroy(pa_gpu_ctx* ctx);
+const char* pa_gpu_last_error(const pa_gpu_ctx* ctx);
+pa_status   pa_gpu_synchronize(pa_gpu_ctx* ctx);
+/* Device name and compute-unit count (for reports). */
+pa_status   pa_gpu_device_info(pa_gpu_ctx* ctx, char* name, size_t name_len, int* compute_units,
+                               size_t* free_bytes, size_t* total_bytes);
+
+/* Device memory and copies (stream-ordered on the context stream). */
+pa_status   pa_gpu_malloc(pa_gpu_ctx* ctx, size_t bytes, void** d_out);
+pa_status   pa_gpu_free(pa_gpu_ctx* ctx, void* d_ptr);
+pa_status   pa_gpu_memcpy_h2d(pa_gpu_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+pa_status   pa_gpu_memcpy_d2h(pa_gpu_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+pa_status   pa_gpu_memcpy_d2d(pa_gpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+
+/* Streaming device copy by a plain kernel (float4 loads and stores), e.g.
+ * to calibrate hardware byte counters against a known transfer. */
+pa_status   pa_gpu_copy(pa_gpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+
+/* Kernel timing: when enabled, every kernel launch of the library is
+ * bracketed with HIP events on the context stream; per-kernel totals are
+ * read with pa_gpu_kernel_time (name as in the kernel table, e.g.
+ * "trace", "prune_counted", "index", "color", "flow_map", "ftle"). */
+pa_status   pa_gpu_timing_enable(pa_gpu_ctx* ctx, int enable);
+pa_status   pa_gpu_timing_reset(pa_gpu_ctx* ctx);
+pa_status   pa_gpu_kernel_time(pa_gpu_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches);
+
+/* ------------------------------------------------------------------- fields */
+/* Block upload.  Replaces the data_io block load into pa::vector_field
+ * (pa/source/stages/data_io.cpp:91-115; vector_field.hpp:14-24): h_xyz is the
+ * reference's boost::multi_array<vector3,3> payload, [d0][d1][d2][3] FP32,
+ * C order (z fastest), dims are the ghosted node counts.  offset, size and
+ * spacing are the vector_field members.  The device copy is stored as padded
+ * 4-component nodes in the requested precision (BF16/FP16 rounded to
+ * nearest even). */
+pa_status   pa_gpu_field_create(pa_gpu_ctx* ctx, const float* h_xyz, const int32_t dims[3], const float offset[3],
+                                const float size[3], const float spacing[3], pa_storage storage,
+                                pa_gpu_field** out);
+/* Same, from an FP32 xyz array already on the device. */
+pa_status   pa_gpu_field_create_device(pa_gpu_ctx* ctx, const float* d_xyz, const int32_t dims[3],
+                                       const float offset[3], const float size[3], const float spacing[3],
+                                       pa_storage storage, pa_gpu_field** out);
+/* Synthesised on the device (no host array): kind 0 = ABC flow
+ * u = (A sin z + C cos y, B sin x + A cos z, C sin y + B cos x) with
+ * params = {A, B, C}; kind 1 = sum of Fourier modes u = sum_m cos(k_m.x +
+ * phi_m) c_m with params = n_modes x {k0,k1,k2,c0,c1,c2,phi}.  Node (i,j,k)
+ * sits at (origin + (i,j,k)) * h, evaluated in FP64, rounded to storage. */
+pa_status   pa_gpu_field_synthesize(pa_gpu_ctx* ctx, int kind, const double* params, int32_t n_params,
+                                    const int32_t dims[3], const int32_t origin[3], double h, const float offset[3],
+                                    const float size[3], const float spacing[3], pa_storage storage,
+                                    pa_gpu_field** out);
+/* Decoded FP32 node values back to the host, [d0][d1][d2][3]. */
+pa_status   pa_gpu_field_read(pa_gpu_ctx* ctx, const pa_gpu_field* field, float* h_xyz);
+pa_status   pa_gpu_field_bytes(const pa_gpu_field* field, size_t* bytes);
+pa_status   pa_gpu_field_destroy(pa_gpu_ctx* ctx, pa_gpu_field* field);
+
+/* vector_field::contains + interpolate (pa/source/math/vector_field.cpp:12-48)
+ * at n points: d_points and d_out are n vec4; d_out = (u, v, w, inside) with
+ * inside 1.0 or 0.0 (u, v, w are 0 when outside). */
+pa_status   pa_gpu_field_interpolate(pa_gpu_ctx* ctx, const pa_gpu_field* field, pa_interp interp,
+                                     const float* d_points, size_t n, float* d_out);
+
+/* -------------------------------------------------------------------- seeds */
+/* pa::seed_generator::generate (pa/source/stages/seed_generator.cpp:9-21;
+ * pa/include/pa/stages/seed_generator.hpp:13): a strided lattice of
+ * int(size/stride) particles per axis, unravelled in C order.  Query the
+ * count with pa_gpu_seed_count first. */
+pa_status   pa_gpu_seed_count(const float size[3], const float stride[3], int32_t dims_out[3], size_t* n_out);
+pa_status   pa_gpu_seed_generate(pa_gpu_ctx* ctx, const float offset[3], const float size[3], const float stride[3],
+                                 int32_t remaining_iterations, int32_t rank, pa_particle* d_out, size_t capacity,
+                                 size_t* n_out);
+
+/* ------------------------------------------------------------------- tracer */
+/* pa::particle_tracer (pa/include/pa/stages/particle_tracer.hpp:39-62). */
+pa_status   pa_gpu_tracer_create(pa_gpu_ctx* ctx, pa_gpu_tracer** out);
+pa_status   pa_gpu_tracer_destroy(pa_gpu_tracer* tracer);
+/* set_local_vector_field (particle_tracer.hpp:46). */
+pa_status   pa_gpu_tracer_set_local_vector_field(pa_gpu_tracer* tracer, const pa_gpu_field* field);
+/* set_neighbor_vector_fields (particle_tracer.hpp:47) plus the neighbour
+ * ranks of partitioner::neighbor_rank_info (partitioner.hpp:39); a rank of
+ * -1 means no neighbour across that face; a NULL field means the block is
+ * not preloaded (the reference loads neighbours only with load balancing,
+ * pars/source/pipeline.cpp:96-98). */
+pa_status   pa_gpu_tracer_set_neighbor_vector_fields(pa_gpu_tracer* tracer, const pa_gpu_field* const fields[6],
+                                                     const int32_t neighbor_rank[6]);
+/* set_integrator (particle_tracer.hpp:48): EULER, RUNGE_KUTTA_4 and
+ * CLASSIC_RK4 are supported; the other odeint steppers return
+ * PA_E_UNSUPPORTED. */
+pa_status   pa_gpu_tracer_set_integrator(pa_gpu_tracer* tracer, pa_integrator integrator);
+/* set_step_size (particle_tracer.hpp:49). */
+pa_status   pa_gpu_tracer_set_step_size(pa_gpu_tracer* tracer, float step_size);
+/* Extension: stop rule (default PA_STOP_ISZERO, the reference's). */
+pa_status   pa_gpu_tracer_set_stop_rule(pa_gpu_tracer* tracer, pa_stop_rule rule, float eps);
+/* Extension: interpolation weights (default PA_INTERP_REFERENCE). */
+pa_status   pa_gpu_tracer_set_interpolation(pa_gpu_tracer* tracer, pa_interp interp);
+/* Union-of-7 topology: bit d of face_mask[k] set when neighbour block k has
+ * its own neighbour across face d (partitioner.cpp:47-52 order). */
+pa_status   pa_gpu_tracer_set_neighbor_face_masks(pa_gpu_tracer* tracer, const int32_t face_mask[6]);
+
+/* compute_round_info (pa/source/stages/particle_tracer.cpp:210-234).
+ * Rejects an empty set or a non-positive maximum (undefined in the
+ * reference). */
+pa_status   pa_gpu_tracer_compute_round_info(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
+                                             size_t integral_curve_offset, pa_round_info* out);
+/* allocate (particle_tracer.cpp:235-242): fill count vertices with
+ * invalid_vertex (-1,-1,-1,0). */
+pa_status   pa_gpu_fill_invalid(pa_gpu_ctx* ctx, float* d_vertices, size_t count);
+/* initialize + trace (particle_tracer.cpp:243-337): one trace round.
+ * d_chunks[c] (host array of integral_curve_count device pointers) holds the
+ * round's chunk c with the size given by info; slots the reference leaves as
+ * invalid_vertex are not written, so callers that need the reference's
+ * pre-prune contents fill them first with pa_gpu_fill_invalid.
+ * d_vertex_counts[p] (optional) receives the slots written for particle p
+ * (seed through termination vertex); handoff (optional) receives the two
+ * hand-off maps; total_steps (optional) the number of integrator steps. */
+pa_status   pa_gpu_tracer_trace(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
+                                const pa_round_info* info, float* const* d_chunks, int32_t* d_vertex_counts,
+                                pa_handoff* handoff, uint64_t* total_steps);
+
+/* Union-of-7 trace (no reference counterpart as one call; equal to the
+ * reference's round loop restricted to the own block and its six preloaded
+ * neighbours, pars/source/pipeline.cpp:153-209): a particle handed to a
+ * preloaded neighbour continues there at once, and one handed back from a
+ * neighbour continues in the own block.  Each seed owns region_slots
+ * consecutive slots of d_vertices (n * region_slots vec4); its segments are
+ * written one after another, each [start, stepped..., termination].
+ * d_vertex_counts[p] = slots used by seed p.  A seed whose segments would
+ * overflow its region is stopped at the segment boundary and written to
+ * d_continuation (capacity n; vector_field_index = block, original_voxel[0]
+ * = seed index), *n_continuation counts them; tracing them again with this
+ * function continues them exactly.  region_slots must be >= the maximum
+ * remaining_iterations.  d_endpoints (optional, n vec4) receives each seed's
+ * last vertex before termination in this call, d_seed_steps (optional) its
+ * integrator steps in this call. */
+pa_status   pa_gpu_tracer_trace_union(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
+                                      size_t region_slots, float* d_vertices, int32_t* d_vertex_counts,
+                                      pa_particle* d_continuation, size_t* n_continuation, float* d_endpoints,
+                                      int32_t* d_seed_steps, uint64_t* total_steps);
+
+/* prune (particle_tracer.cpp:402-409): order-preserving removal of
+ * invalid_vertex from one chunk.  *n_out receives the kept count. */
+pa_status   pa_gpu_prune(pa_gpu_ctx* ctx, const float* d_vertices, size_t n, float* d_out, size_t* n_out);
+/* prune of a traced round without the invalid fill: particle p's
+ * d_vertex_counts[p] slots, starting at slot p*stride_slots of the
+ * concatenated d_slots, are packed in particle order into d_out.
+ * d_offsets (optional, n+1) receives the exclusive scan. */
+pa_status   pa_gpu_prune_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t stride_slots,
+                                 const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,
+                                 size_t* n_out, int64_t* d_offsets);
+
+/* pa::index_generator::generate (pa/source/stages/index_generator.cpp:10-24):
+ * index k is emitted iff neither vertex k nor k+1 is the termination vertex;
+ * d_indices needs capacity n.  Rejects n == 0 (size()-1 underflows in the
+ * reference). */
+pa_status   pa_gpu_index_generate(pa_gpu_ctx* ctx, const float* d_vertices, size_t n, int32_t* d_indices,
+                                  size_t* n_out);
+
+/* pa::color_generator::generate (pa/source/stages/color_generator.cpp:16-40).
+ * d_colors has n_vertices vec4; colors the reference leaves uninitialised are
+ * left untouched here (callers zero them for comparisons). */
+pa_status   pa_gpu_color_generate(pa_gpu_ctx* ctx, const float* d_vertices, size_t n_vertices,
+                                  const int32_t* d_indices, size_t n_indices, pa_color_mode mode, float parameter,
+                                  float* d_colors);
+
+/* ----------------------------------------------------------- advector / FTLE */
+/* pa::particle_advector::advect, one round in place over one block
+ * (pa/source/stages/particle_advector.cpp:61-141).  d_state[p]: 1 =
+ * inactive (returned to original_rank), 2 = handed to neighbour slot
+ * d_handoff_slot[p], 3 = never stepped (remaining_iterations <= 1; the
+ * reference drops it).  Uses the tracer's local field, neighbour ranks,
+ * integrator, step size and stop rule. */
+pa_status   pa_gpu_advect(pa_gpu_tracer* tracer, pa_particle* d_particles, size_t n, int32_t* d_state,
+                          int32_t* d_handoff_slot, uint64_t* total_steps);
+
+/* pa::flow_map_generator::generate (pa/source/stages/flow_map_generator.cpp:
+ * 12-52) for a block without neighbours: dims = block_size * scale,
+ * spacing / scale, one seed per voxel, advected in place, final xyz written
+ * to d_flow_map[(x*d1+y)*d2+z][3] (voxels of lost particles keep the seed
+ * position).  Seeds are generated inside the kernel; no particle array is
+ * materialised. */
+pa_status   pa_gpu_flow_map_generate(pa_gpu_tracer* tracer, const int32_t block_size[3], uint64_t iterations,
+                                     float resolution_scale, int32_t rank, float* d_flow_map, int32_t dims_out[3],
+                                     uint64_t* total_steps);
+
+/* pa::ftle_map_generator::generate (pa/source/stages/ftle_map_generator.cpp:
+ * 22-60) in textbook form, all in FP64 on the device: flow-map Jacobian by
+ * central differences (one-sided at the lattice border), C = J^T J, symmetric
+ * 3x3 eigenvalues, then ln(sqrt(lambda_max))/|T| or fractional anisotropy,
+ * stored FP32.  d_lambda_max (optional) receives lambda_max in FP64. */
+pa_status   pa_gpu_ftle_generate(pa_gpu_ctx* ctx, const float* d_flow_map, const int32_t dims[3],
+                                 const float spacing[3], double time, pa_ftle_mode mode, float* d_ftle,
+                                 double* d_lambda_max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PA_GPU_H */

@@ -1,0 +1,567 @@
+"""pars_amd — MI355X (gfx950) implementation of pars' per-process advection path.
+
+Python mirror of the reference's ``pa`` stage interface (acdemiralp/pars,
+``pa/include/pa/stages/*.hpp``) over the C ABI of ``libpa_gpu.so``
+(``include/pa_gpu.h``).  Stage and method names follow the reference:
+``seed_generator.generate``, ``particle_tracer`` (``set_*``,
+``compute_round_info``, ``allocate``, ``initialize``/``trace``, ``prune``),
+``index_generator.generate``, ``color_generator.generate``,
+``flow_map_generator`` and ``ftle_map_generator``.
+
+All data stays in device memory (``DeviceBuffer``); the HIP library is the
+only compute path.  Importing this package fails loudly when the library has
+not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpa_gpu.so")
+
+PARTICLE_DTYPE = np.dtype(
+    [
+        ("position", "<f4", (4,)),
+        ("remaining_iterations", "<i4"),
+        ("vector_field_index", "<i4"),
+        ("original_rank", "<i4"),
+        ("original_voxel", "<i4", (3,)),
+        ("reserved_", "<i4", (2,)),
+    ]
+)
+assert PARTICLE_DTYPE.itemsize == 48
+
+# pa_status
+OK, E_INVALID, E_UNSUPPORTED, E_DEVICE, E_NOMEM, E_CAPACITY, E_STATE = range(7)
+# pa_storage
+FP32, BF16, FP16 = 0, 1, 2
+# pa_integrator (pa::variant_integrator order) + extension
+EULER, MODIFIED_MIDPOINT, RUNGE_KUTTA_4, CASH_KARP_54, DORMAND_PRINCE_5, FEHLBERG_78, AB2, ABM2 = range(8)
+CLASSIC_RK4 = 100
+INTEGRATORS = {
+    "euler": EULER, "modified_midpoint": MODIFIED_MIDPOINT, "runge_kutta_4": RUNGE_KUTTA_4,
+    "runge_kutta_cash_karp_54": CASH_KARP_54, "runge_kutta_dormand_prince_5": DORMAND_PRINCE_5,
+    "runge_kutta_fehlberg_78": FEHLBERG_78, "adams_bashforth_2": AB2, "adams_bashforth_moulton_2": ABM2,
+    "classic_rk4": CLASSIC_RK4,
+}
+STOP_ISZERO, STOP_NORM = 0, 1
+INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
+TERMINATION_VERTEX = np.array([-2.0, -2.0, -2.0, 0.0], np.float32)
+INVALID_VERTEX = np.array([-1.0, -1.0, -1.0, 0.0], np.float32)
+INVALID_INDEX = -1
+
+
+class PaError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__("pa_gpu status %d: %s" % (status, message))
+        self.status = status
+
+
+class RoundInfo(C.Structure):
+    """particle_tracer::round_info (particle_tracer.hpp:26-37) without the maps."""
+
+    _fields_ = [
+        ("maximum_remaining_iterations", C.c_uint64),
+        ("vertices_per_integral_curve", C.c_uint64),
+        ("vertices_per_integral_curve_last", C.c_uint64),
+        ("integral_curve_offset", C.c_uint64),
+        ("integral_curve_count", C.c_uint64),
+    ]
+
+    def chunk_sizes(self):
+        n = int(self.integral_curve_count)
+        return [int(self.vertices_per_integral_curve) if c != n - 1 else int(self.vertices_per_integral_curve_last)
+                for c in range(n)]
+
+
+class _Handoff(C.Structure):
+    _fields_ = [
+        ("d_particles", C.c_void_p),
+        ("d_sources", C.c_void_p),
+        ("capacity", C.c_size_t),
+        ("counts", (C.c_size_t * 6) * 2),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libpa_gpu.so.  Raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("pars_amd: %s is missing; run `python -m pars_amd.build` (or __graft_entry__.build())"
+                              % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.pa_gpu_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+def _i3(v):
+    return (C.c_int32 * 3)(*[int(x) for x in v])
+
+
+class Context:
+    """A device context (pa_gpu_ctx): one HIP stream on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        s = lib().pa_gpu_ctx_create(C.c_int(device), C.byref(self._h))
+        if s != OK:
+            raise PaError(s, "pa_gpu_ctx_create(%d) failed (no usable GPU?)" % device)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, status: int, what: str = ""):
+        if status != OK:
+            msg = lib().pa_gpu_last_error(self._h).decode()
+            raise PaError(status, (what + ": " if what else "") + msg)
+
+    def synchronize(self):
+        self.check(lib().pa_gpu_synchronize(self._h), "synchronize")
+
+    def close(self):
+        if self._h:
+            lib().pa_gpu_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        cu = C.c_int()
+        free = C.c_size_t()
+        total = C.c_size_t()
+        self.check(lib().pa_gpu_device_info(self._h, name, C.c_size_t(256), C.byref(cu), C.byref(free),
+                                            C.byref(total)))
+        return dict(name=name.value.decode(), compute_units=cu.value, free_bytes=free.value, total_bytes=total.value)
+
+    # timing -------------------------------------------------------------
+    def timing(self, enable: bool = True):
+        self.check(lib().pa_gpu_timing_enable(self._h, C.c_int(int(enable))))
+
+    def timing_reset(self):
+        self.check(lib().pa_gpu_timing_reset(self._h))
+
+    def kernel_time(self, name: str):
+        ms = C.c_double()
+        n = C.c_uint64()
+        self.check(lib().pa_gpu_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # memory ---------------------------------------------------------------
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, array: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(array)
+        b = DeviceBuffer(self, a.nbytes)
+        b.upload(a)
+        return b
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (pa_gpu_malloc / pa_gpu_free)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self._p = C.c_void_p()
+        ctx.check(lib().pa_gpu_malloc(ctx.handle, C.c_size_t(max(self.nbytes, 16)), C.byref(self._p)), "malloc")
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def at(self, byte_offset: int) -> int:
+        return self._p.value + int(byte_offset)
+
+    def upload(self, array: np.ndarray, byte_offset: int = 0):
+        a = np.ascontiguousarray(array)
+        assert byte_offset + a.nbytes <= self.nbytes
+        self.ctx.check(lib().pa_gpu_memcpy_h2d(self.ctx.handle, C.c_void_p(self.at(byte_offset)),
+                                               a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes)), "h2d")
+
+    def download(self, dtype, count: Optional[int] = None, byte_offset: int = 0, shape=None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - byte_offset) // dtype.itemsize
+        out = np.empty(count, dtype)
+        if count:
+            self.ctx.check(lib().pa_gpu_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(C.c_void_p),
+                                                   C.c_void_p(self.at(byte_offset)),
+                                                   C.c_size_t(count * dtype.itemsize)), "d2h")
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self._p.value:
+            lib().pa_gpu_free(self.ctx.handle, self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class VectorField:
+    """A block resident in HBM (pa::vector_field, vector_field.hpp:14-24)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, dims, offset, size, spacing, storage):
+        self.ctx = ctx
+        self._h = handle
+        self.dims = tuple(int(d) for d in dims)
+        self.offset = np.asarray(offset, np.float32)
+        self.size = np.asarray(size, np.float32)
+        self.spacing = np.asarray(spacing, np.float32)
+        self.storage = storage
+
+    @classmethod
+    def from_host(cls, ctx: Context, xyz: np.ndarray, offset, size, spacing, storage=FP32):
+        """Upload [d0][d1][d2][3] FP32 data (data_io.cpp:91-115 replacement)."""
+        a = np.ascontiguousarray(xyz, np.float32)
+        assert a.ndim == 4 and a.shape[3] == 3
+        h = C.c_void_p()
+        ctx.check(lib().pa_gpu_field_create(ctx.handle, a.ctypes.data_as(C.c_void_p), _i3(a.shape[:3]), _f3(offset),
+                                            _f3(size), _f3(spacing), C.c_int(storage), C.byref(h)), "field_create")
+        return cls(ctx, h, a.shape[:3], offset, size, spacing, storage)
+
+    @classmethod
+    def synthesize(cls, ctx: Context, kind: str, params, dims, origin, h, offset, size, spacing, storage=FP32):
+        """ABC flow (params A, B, C) or Fourier modes (rows k0 k1 k2 c0 c1 c2 phi) on the device."""
+        p = np.ascontiguousarray(params, np.float64).ravel()
+        k = {"abc": 0, "fourier": 1}[kind]
+        hh = C.c_void_p()
+        ctx.check(lib().pa_gpu_field_synthesize(ctx.handle, C.c_int(k), p.ctypes.data_as(C.c_void_p),
+                                                C.c_int32(len(p)), _i3(dims), _i3(origin), C.c_double(h),
+                                                _f3(offset), _f3(size), _f3(spacing), C.c_int(storage),
+                                                C.byref(hh)), "field_synthesize")
+        return cls(ctx, hh, dims, offset, size, spacing, storage)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def read(self) -> np.ndarray:
+        out = np.empty(self.dims + (3,), np.float32)
+        self.ctx.check(lib().pa_gpu_field_read(self.ctx.handle, self._h, out.ctypes.data_as(C.c_void_p)),
+                       "field_read")
+        return out
+
+    def interpolate(self, points: np.ndarray, interp: int = INTERP_REFERENCE) -> np.ndarray:
+        """vector_field::contains + interpolate at points (n x 4) -> (n x 4): u, v, w, inside."""
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 4)
+        n = len(pts)
+        d_in = self.ctx.upload(pts)
+        d_out = self.ctx.alloc(max(n, 1) * 16)
+        self.ctx.check(lib().pa_gpu_field_interpolate(self.ctx.handle, self._h, C.c_int(interp),
+                                                      C.c_void_p(d_in.ptr), C.c_size_t(n), C.c_void_p(d_out.ptr)),
+                       "field_interpolate")
+        return d_out.download(np.float32, 4 * n).reshape(n, 4)
+
+    def nbytes(self) -> int:
+        b = C.c_size_t()
+        lib().pa_gpu_field_bytes(self._h, C.byref(b))
+        return b.value
+
+    def release(self):
+        if self._h and self._h.value:
+            lib().pa_gpu_field_destroy(self.ctx.handle, self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Particles:
+    """A device array of pa_particle (48 B each)."""
+
+    def __init__(self, buf: DeviceBuffer, n: int):
+        self.buf = buf
+        self.n = int(n)
+
+    @property
+    def ptr(self):
+        return self.buf.ptr
+
+    @classmethod
+    def from_host(cls, ctx: Context, particles: np.ndarray):
+        p = np.ascontiguousarray(particles, PARTICLE_DTYPE)
+        return cls(ctx.upload(p) if len(p) else ctx.alloc(48), len(p))
+
+    def download(self) -> np.ndarray:
+        return self.buf.download(PARTICLE_DTYPE, self.n)
+
+
+class seed_generator:
+    """pa::seed_generator (seed_generator.hpp:10-14; seed_generator.cpp:9-21)."""
+
+    @staticmethod
+    def count(size, stride):
+        dims = (C.c_int32 * 3)()
+        n = C.c_size_t()
+        s = lib().pa_gpu_seed_count(_f3(size), _f3(stride), dims, C.byref(n))
+        if s != OK:
+            raise PaError(s, "seed_count")
+        return int(n.value), tuple(dims)
+
+    @staticmethod
+    def generate(ctx: Context, offset, size, stride, remaining_iterations: int, rank: int = 0) -> Particles:
+        n, _ = seed_generator.count(size, stride)
+        buf = ctx.alloc(max(n, 1) * 48)
+        got = C.c_size_t()
+        ctx.check(lib().pa_gpu_seed_generate(ctx.handle, _f3(offset), _f3(size), _f3(stride),
+                                             C.c_int32(remaining_iterations), C.c_int32(rank), C.c_void_p(buf.ptr),
+                                             C.c_size_t(max(n, 1)), C.byref(got)), "seed_generate")
+        return Particles(buf, got.value)
+
+
+class particle_tracer:
+    """pa::particle_tracer (particle_tracer.hpp:39-62) on the GPU."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        ctx.check(lib().pa_gpu_tracer_create(ctx.handle, C.byref(self._h)), "tracer_create")
+        self._keep = []
+
+    def __del__(self):
+        try:
+            lib().pa_gpu_tracer_destroy(self._h)
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_local_vector_field(self, field: VectorField):
+        self._local = field
+        self.ctx.check(lib().pa_gpu_tracer_set_local_vector_field(self._h, field.handle if field else None))
+
+    def set_neighbor_vector_fields(self, fields: Sequence[Optional[VectorField]], neighbor_rank: Sequence[int]):
+        arr = (C.c_void_p * 6)(*[(f.handle.value if f is not None else None) for f in fields])
+        self._nbrs = list(fields)
+        self.ctx.check(lib().pa_gpu_tracer_set_neighbor_vector_fields(self._h, arr, (C.c_int32 * 6)(*neighbor_rank)))
+
+    def set_neighbor_face_masks(self, masks: Sequence[int]):
+        self.ctx.check(lib().pa_gpu_tracer_set_neighbor_face_masks(self._h, (C.c_int32 * 6)(*masks)))
+
+    def set_integrator(self, integrator):
+        if isinstance(integrator, str):
+            integrator = INTEGRATORS[integrator]
+        self.ctx.check(lib().pa_gpu_tracer_set_integrator(self._h, C.c_int(integrator)), "set_integrator")
+
+    def set_step_size(self, step_size: float):
+        self.ctx.check(lib().pa_gpu_tracer_set_step_size(self._h, C.c_float(step_size)))
+
+    def set_interpolation(self, interp: int):
+        self.ctx.check(lib().pa_gpu_tracer_set_interpolation(self._h, C.c_int(interp)), "set_interpolation")
+
+    def set_stop_rule(self, rule: int, eps: float = 0.0):
+        self.ctx.check(lib().pa_gpu_tracer_set_stop_rule(self._h, C.c_int(rule), C.c_float(eps)))
+
+    # sub-methods (particle_tracer.hpp:55-61) ---------------------------------
+    def compute_round_info(self, particles: Particles, integral_curve_offset: int = 0) -> RoundInfo:
+        info = RoundInfo()
+        self.ctx.check(lib().pa_gpu_tracer_compute_round_info(self._h, C.c_void_p(particles.ptr),
+                                                              C.c_size_t(particles.n),
+                                                              C.c_size_t(integral_curve_offset), C.byref(info)),
+                       "compute_round_info")
+        return info
+
+    def allocate(self, info: RoundInfo, fill_invalid: bool = True):
+        chunks = []
+        for sz in info.chunk_sizes():
+            b = self.ctx.alloc(max(sz, 1) * 16)
+            if fill_invalid and sz:
+                self.ctx.check(lib().pa_gpu_fill_invalid(self.ctx.handle, C.c_void_p(b.ptr), C.c_size_t(sz)))
+            chunks.append((b, sz))
+        return chunks
+
+    def trace(self, particles: Particles, chunks, info: RoundInfo, want_counts=True, want_handoff=True):
+        """initialize + trace of one round.  Returns dict(counts, handoff, handoff_counts, total_steps)."""
+        table = (C.c_void_p * max(1, len(chunks)))(*[c[0].ptr for c in chunks])
+        counts = self.ctx.alloc(max(particles.n, 1) * 4) if want_counts else None
+        hand = None
+        h = None
+        if want_handoff:
+            hand = self.ctx.alloc(max(particles.n, 1) * 48)
+            src = self.ctx.alloc(max(particles.n, 1) * 8)
+            h = _Handoff(hand.ptr, src.ptr, max(particles.n, 1))
+        steps = C.c_uint64()
+        self.ctx.check(lib().pa_gpu_tracer_trace(self._h, C.c_void_p(particles.ptr), C.c_size_t(particles.n),
+                                                 C.byref(info), table,
+                                                 C.c_void_p(counts.ptr) if counts else None,
+                                                 C.byref(h) if h is not None else None, C.byref(steps)), "trace")
+        out = dict(total_steps=int(steps.value), counts=counts)
+        if h is not None:
+            hc = np.array([[h.counts[k][s] for s in range(6)] for k in range(2)], np.int64)
+            tot = int(hc.sum())
+            out["handoff_counts"] = hc
+            out["handoff"] = hand.download(PARTICLE_DTYPE, tot) if tot else np.zeros(0, PARTICLE_DTYPE)
+            out["handoff_sources"] = src.download(np.int64, tot) if tot else np.zeros(0, np.int64)
+        return out
+
+    def trace_union(self, particles: Particles, region_slots: int, buffers=None, endpoints=False):
+        """Union-of-7 trace; buffers (from union_buffers) may be reused across calls."""
+        b = buffers or self.union_buffers(particles.n, region_slots, endpoints)
+        ncont = C.c_size_t()
+        steps = C.c_uint64()
+        self.ctx.check(lib().pa_gpu_tracer_trace_union(
+            self._h, C.c_void_p(particles.ptr), C.c_size_t(particles.n), C.c_size_t(region_slots),
+            C.c_void_p(b["vertices"].ptr), C.c_void_p(b["counts"].ptr), C.c_void_p(b["cont"].ptr), C.byref(ncont),
+            C.c_void_p(b["endpoints"].ptr) if b.get("endpoints") else None,
+            C.c_void_p(b["seed_steps"].ptr) if b.get("seed_steps") else None, C.byref(steps)), "trace_union")
+        return dict(vertices=b["vertices"], counts=b["counts"], continuation=Particles(b["cont"], ncont.value),
+                    endpoints=b.get("endpoints"), seed_steps=b.get("seed_steps"), total_steps=int(steps.value))
+
+    def union_buffers(self, n: int, region_slots: int, endpoints: bool = False):
+        n1 = max(n, 1)
+        b = dict(vertices=self.ctx.alloc(n1 * region_slots * 16), counts=self.ctx.alloc(n1 * 4),
+                 cont=self.ctx.alloc(n1 * 48))
+        if endpoints:
+            b["endpoints"] = self.ctx.alloc(n1 * 16)
+            b["seed_steps"] = self.ctx.alloc(n1 * 4)
+        return b
+
+    def advect(self, particles: Particles):
+        """particle_advector::advect, one round in place (particle_advector.cpp:61-141)."""
+        state = self.ctx.alloc(max(particles.n, 1) * 4)
+        slot = self.ctx.alloc(max(particles.n, 1) * 4)
+        steps = C.c_uint64()
+        self.ctx.check(lib().pa_gpu_advect(self._h, C.c_void_p(particles.ptr), C.c_size_t(particles.n),
+                                           C.c_void_p(state.ptr), C.c_void_p(slot.ptr), C.byref(steps)), "advect")
+        return state.download(np.int32, particles.n), slot.download(np.int32, particles.n), int(steps.value)
+
+    @staticmethod
+    def prune(ctx: Context, vertices: DeviceBuffer, n: int):
+        """particle_tracer::prune of one chunk (particle_tracer.cpp:402-409)."""
+        out = ctx.alloc(max(n, 1) * 16)
+        m = C.c_size_t()
+        ctx.check(lib().pa_gpu_prune(ctx.handle, C.c_void_p(vertices.ptr), C.c_size_t(n), C.c_void_p(out.ptr),
+                                     C.byref(m)), "prune")
+        return out, m.value
+
+    @staticmethod
+    def prune_counted(ctx: Context, slots: int, stride_slots: int, counts: DeviceBuffer, n: int, capacity=None):
+        """Pack each particle's counted slots (no invalid fill needed)."""
+        capacity = capacity or max(1, n * stride_slots)
+        out = ctx.alloc(capacity * 16)
+        m = C.c_size_t()
+        offs = ctx.alloc((n + 1) * 8)
+        ctx.check(lib().pa_gpu_prune_counted(ctx.handle, C.c_void_p(slots), C.c_size_t(stride_slots),
+                                             C.c_void_p(counts.ptr), C.c_size_t(n), C.c_void_p(out.ptr),
+                                             C.c_size_t(capacity), C.byref(m), C.c_void_p(offs.ptr)), "prune_counted")
+        return out, m.value, offs
+
+
+class index_generator:
+    """pa::index_generator (index_generator.cpp:10-24)."""
+
+    @staticmethod
+    def generate(ctx: Context, vertices: DeviceBuffer, n: int):
+        out = ctx.alloc(max(n, 1) * 4)
+        m = C.c_size_t()
+        ctx.check(lib().pa_gpu_index_generate(ctx.handle, C.c_void_p(vertices.ptr), C.c_size_t(n),
+                                              C.c_void_p(out.ptr), C.byref(m)), "index_generate")
+        return out, m.value
+
+
+class color_generator:
+    """pa::color_generator (color_generator.hpp:10-32; color_generator.cpp:16-98)."""
+
+    hsl_constant_s, hsl_constant_l, hsv_constant_s, hsv_constant_v, rgb = range(5)
+    magnitude = 100
+    MODES = {"hsl_constant_s": 0, "hsl_constant_l": 1, "hsv_constant_s": 2, "hsv_constant_v": 3, "rgb": 4,
+             "magnitude": 100}
+
+    @staticmethod
+    def generate(ctx: Context, vertices: DeviceBuffer, n_vertices: int, indices: DeviceBuffer, n_indices: int,
+                 mode=4, constant_parameter: float = 0.5, colors: Optional[DeviceBuffer] = None):
+        if isinstance(mode, str):
+            mode = color_generator.MODES[mode]
+        if colors is None:
+            colors = ctx.upload(np.zeros((max(n_vertices, 1), 4), np.float32))
+        ctx.check(lib().pa_gpu_color_generate(ctx.handle, C.c_void_p(vertices.ptr), C.c_size_t(n_vertices),
+                                              C.c_void_p(indices.ptr), C.c_size_t(n_indices), C.c_int(mode),
+                                              C.c_float(constant_parameter), C.c_void_p(colors.ptr)),
+                  "color_generate")
+        return colors
+
+
+class flow_map_generator:
+    """pa::flow_map_generator (flow_map_generator.hpp:12-28) for one block without neighbours."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.tracer = particle_tracer(ctx)
+        self.tracer.set_integrator(EULER)  # particle_advector.hpp:50 default
+
+    def set_vector_field(self, field: VectorField):
+        self.field = field
+        self.tracer.set_local_vector_field(field)
+
+    def set_integrator(self, integrator):
+        self.tracer.set_integrator(integrator)
+
+    def set_step_size(self, step_size: float):
+        self.tracer.set_step_size(step_size)
+
+    def generate(self, block_size, iterations: int, resolution_scale: float = 1.0, rank: int = 0):
+        dims = [int(np.float32(b) * np.float32(resolution_scale)) for b in block_size]
+        out = self.ctx.alloc(max(1, dims[0] * dims[1] * dims[2]) * 12)
+        d = (C.c_int32 * 3)()
+        steps = C.c_uint64()
+        self.ctx.check(lib().pa_gpu_flow_map_generate(self.tracer.handle, _i3(block_size), C.c_uint64(iterations),
+                                                      C.c_float(resolution_scale), C.c_int32(rank),
+                                                      C.c_void_p(out.ptr), d, C.byref(steps)), "flow_map")
+        assert list(d) == dims
+        spacing = self.field.spacing / np.float32(resolution_scale)
+        return out, tuple(dims), spacing, int(steps.value)
+
+
+class ftle_map_generator:
+    """pa::ftle_map_generator (ftle_map_generator.hpp:13-40), textbook FP64 form."""
+
+    regular, fractional_anisotropy = 0, 1
+
+    def __init__(self, ctx: Context, mode: int = 0):
+        self.ctx = ctx
+        self.mode = mode
+        self.time = 1.0
+
+    def set_flow_map(self, flow_map: DeviceBuffer, dims, spacing):
+        self.flow_map, self.dims, self.spacing = flow_map, tuple(dims), np.asarray(spacing, np.float32)
+
+    def set_time(self, time: float):
+        self.time = float(time)
+
+    def generate(self, want_lambda=False):
+        n = self.dims[0] * self.dims[1] * self.dims[2]
+        out = self.ctx.alloc(max(n, 1) * 4)
+        lm = self.ctx.alloc(max(n, 1) * 8) if want_lambda else None
+        self.ctx.check(lib().pa_gpu_ftle_generate(self.ctx.handle, C.c_void_p(self.flow_map.ptr), _i3(self.dims),
+                                                  _f3(self.spacing), C.c_double(self.time), C.c_int(self.mode),
+                                                  C.c_void_p(out.ptr), C.c_void_p(lm.ptr) if lm else None), "ftle")
+        return out, lm

@@ -1,0 +1,248 @@
+// Context, memory, timing and error plumbing of the C ABI.
+#include <cstring>
+
+#include "pa_internal.hpp"
+
+namespace pa_amd {
+
+pa_status fail(pa_gpu_ctx* ctx, pa_status s, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  return s;
+}
+
+pa_status check_hip(pa_gpu_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(ctx, e == hipErrorOutOfMemory ? PA_E_NOMEM : PA_E_DEVICE, m);
+}
+
+pa_status scratch(pa_gpu_ctx* ctx, size_t bytes, void** out) {
+  if (bytes > ctx->scratch_bytes) {
+    if (ctx->scratch) PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) PA_HIP(ctx, hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes + bytes / 4;
+    PA_HIP(ctx, hipMalloc(&ctx->scratch, want));
+    ctx->scratch_bytes = want;
+  }
+  *out = ctx->scratch;
+  return PA_OK;
+}
+
+static hipEvent_t take_event(pa_gpu_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Fold finished event pairs into the per-kernel totals.
+static void drain(pa_gpu_ctx* ctx, bool wait) {
+  std::vector<pa_gpu_ctx::timed> keep;
+  for (auto& t : ctx->pending) {
+    if (!wait && hipEventQuery(t.b) != hipSuccess) {
+      keep.push_back(t);
+      continue;
+    }
+    hipEventSynchronize(t.b);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, t.a, t.b);
+    auto& slot = ctx->kernel_ms[t.name];
+    slot.first += double(ms);
+    slot.second += 1;
+    ctx->event_pool.push_back(t.a);
+    ctx->event_pool.push_back(t.b);
+  }
+  ctx->pending.swap(keep);
+}
+
+void launch_begin(pa_gpu_ctx* ctx, const char* name) {
+  if (!ctx->timing) return;
+  pa_gpu_ctx::timed t;
+  t.a = take_event(ctx);
+  t.b = take_event(ctx);
+  t.name = name;
+  if (!t.a || !t.b) return;
+  hipEventRecord(t.a, ctx->stream);
+  ctx->pending.push_back(t);
+}
+
+void launch_end(pa_gpu_ctx* ctx) {
+  if (!ctx->timing || ctx->pending.empty()) return;
+  hipEventRecord(ctx->pending.back().b, ctx->stream);
+  if (ctx->pending.size() > 256) drain(ctx, true);
+}
+
+}  // namespace pa_amd
+
+using namespace pa_amd;
+
+extern "C" {
+
+pa_status pa_gpu_ctx_create(int device, pa_gpu_ctx** out) {
+  if (!out) return PA_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PA_E_DEVICE;
+  if (device < 0 || device >= count) return PA_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return PA_E_DEVICE;
+  auto* ctx = new pa_gpu_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return PA_E_DEVICE;
+  }
+  if (hipMalloc(&ctx->counters, 4096) != hipSuccess) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PA_E_NOMEM;
+  }
+  if (hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+    hipFree(ctx->counters);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PA_E_NOMEM;
+  }
+  *out = ctx;
+  return PA_OK;
+}
+
+pa_status pa_gpu_ctx_destroy(pa_gpu_ctx* ctx) {
+  if (!ctx) return PA_E_INVALID;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  drain(ctx, true);
+  for (auto e : ctx->event_pool) hipEventDestroy(e);
+  if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->counters) hipFree(ctx->counters);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PA_OK;
+}
+
+const char* pa_gpu_last_error(const pa_gpu_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+pa_status pa_gpu_synchronize(pa_gpu_ctx* ctx) {
+  if (!ctx) return PA_E_INVALID;
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PA_OK;
+}
+
+pa_status pa_gpu_device_info(pa_gpu_ctx* ctx, char* name, size_t name_len, int* compute_units, size_t* free_bytes,
+                             size_t* total_bytes) {
+  if (!ctx) return PA_E_INVALID;
+  hipDeviceProp_t prop;
+  PA_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (compute_units) *compute_units = prop.multiProcessorCount;
+  size_t f = 0, t = 0;
+  PA_HIP(ctx, hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return PA_OK;
+}
+
+pa_status pa_gpu_malloc(pa_gpu_ctx* ctx, size_t bytes, void** d_out) {
+  if (!ctx || !d_out) return PA_E_INVALID;
+  *d_out = nullptr;
+  if (bytes == 0) return PA_OK;
+  PA_HIP(ctx, hipMalloc(d_out, bytes));
+  return PA_OK;
+}
+
+pa_status pa_gpu_free(pa_gpu_ctx* ctx, void* d_ptr) {
+  if (!ctx) return PA_E_INVALID;
+  if (!d_ptr) return PA_OK;
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  PA_HIP(ctx, hipFree(d_ptr));
+  return PA_OK;
+}
+
+pa_status pa_gpu_memcpy_h2d(pa_gpu_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+  if (!ctx || (bytes && (!d_dst || !h_src))) return PA_E_INVALID;
+  if (!bytes) return PA_OK;
+  PA_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PA_OK;
+}
+
+pa_status pa_gpu_memcpy_d2h(pa_gpu_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+  if (!ctx || (bytes && (!h_dst || !d_src))) return PA_E_INVALID;
+  if (!bytes) return PA_OK;
+  PA_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return PA_OK;
+}
+
+pa_status pa_gpu_memcpy_d2d(pa_gpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+  if (!ctx || (bytes && (!d_dst || !d_src))) return PA_E_INVALID;
+  if (!bytes) return PA_OK;
+  PA_HIP(ctx, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return PA_OK;
+}
+
+namespace {
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) dst[i] = src[i];
+}
+__global__ void k_copy_tail(const unsigned char* __restrict__ src, unsigned char* __restrict__ dst, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+}  // namespace
+
+pa_status pa_gpu_copy(pa_gpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+  if (!ctx || (bytes && (!d_dst || !d_src))) return PA_E_INVALID;
+  if (!bytes) return PA_OK;
+  if ((reinterpret_cast<uintptr_t>(d_dst) | reinterpret_cast<uintptr_t>(d_src)) & 15)
+    return fail(ctx, PA_E_INVALID, "pa_gpu_copy needs 16-byte aligned pointers");
+  const int64_t n4 = int64_t(bytes / 16);
+  const int64_t tail = int64_t(bytes % 16);
+  launch_begin(ctx, "copy");
+  if (n4) {
+    int64_t g = (n4 + 255) / 256;
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(k_copy, dim3(unsigned(g)), dim3(256), 0, ctx->stream, static_cast<const float4*>(d_src),
+                       static_cast<float4*>(d_dst), n4);
+  }
+  if (tail)
+    hipLaunchKernelGGL(k_copy_tail, dim3(1), dim3(64), 0, ctx->stream,
+                       static_cast<const unsigned char*>(d_src) + n4 * 16, static_cast<unsigned char*>(d_dst) + n4 * 16,
+                       tail);
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+
+pa_status pa_gpu_timing_enable(pa_gpu_ctx* ctx, int enable) {
+  if (!ctx) return PA_E_INVALID;
+  ctx->timing = enable != 0;
+  return PA_OK;
+}
+
+pa_status pa_gpu_timing_reset(pa_gpu_ctx* ctx) {
+  if (!ctx) return PA_E_INVALID;
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  drain(ctx, true);
+  ctx->kernel_ms.clear();
+  return PA_OK;
+}
+
+pa_status pa_gpu_kernel_time(pa_gpu_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches) {
+  if (!ctx || !kernel) return PA_E_INVALID;
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  drain(ctx, true);
+  auto it = ctx->kernel_ms.find(kernel);
+  if (total_ms) *total_ms = it == ctx->kernel_ms.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == ctx->kernel_ms.end() ? 0 : it->second.second;
+  return PA_OK;
+}
+
+}  // extern "C"

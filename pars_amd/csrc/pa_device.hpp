@@ -1,0 +1,252 @@
+// Device-side arithmetic of the advection path.  Every floating-point
+// operation here is a single IEEE-rounded operation in the order of the
+// reference (contraction is off for the whole library; division and square
+// root are the correctly rounded forms), so the FP32 results are
+// bit-identical to the reference's SSE arithmetic on the same inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "pa_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pa_amd {
+
+enum { STEP_RK4 = 0, STEP_EULER = 1, STEP_CLASSIC = 2 };
+enum { END_NORMAL = 0, END_EXIT = 1 };
+
+__device__ __forceinline__ float bf16_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
+__device__ __forceinline__ float f16_lo(uint32_t u) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(u & 0xffffu)));
+}
+__device__ __forceinline__ float f16_hi(uint32_t u) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16)));
+}
+
+// One node of a block, decoded to FP32 (x, y, z); the 4th component is
+// padding.
+template <int S>
+__device__ __forceinline__ void load_node(const void* base, int64_t i, float& x, float& y, float& z) {
+  if constexpr (S == PA_STORAGE_FP32) {
+    const float4 v = static_cast<const float4*>(base)[i];
+    x = v.x; y = v.y; z = v.z;
+  } else {
+    const uint2 u = static_cast<const uint2*>(base)[i];
+    if constexpr (S == PA_STORAGE_BF16) {
+      x = bf16_lo(u.x); y = bf16_hi(u.x); z = bf16_lo(u.y);
+    } else {
+      x = f16_lo(u.x); y = f16_hi(u.x); z = f16_lo(u.y);
+    }
+  }
+}
+
+// Exact fmodf(d, h) for h > 0, given q = d / h (correctly rounded).  The
+// remainder d - n*h is exactly representable once n = floor(|d|/h) is the
+// true quotient; n from floor(|q|) is off by at most one for |q| < 2^22 and
+// the sign of the FMA residual says which way.  Large quotients use the
+// library fmodf (also exact).
+__device__ __forceinline__ float exact_fmod(float d, float h, float q) {
+  const float ax = fabsf(d);
+  const float aq = fabsf(q);
+  if (!(aq < 4194304.0f)) return fmodf(d, h);
+  float n = floorf(aq);
+  float r = __builtin_fmaf(-n, h, ax);
+  if (r < 0.0f) {
+    n -= 1.0f;
+    r = __builtin_fmaf(-n, h, ax);
+  } else if (r >= h) {
+    n += 1.0f;
+    r = __builtin_fmaf(-n, h, ax);
+  }
+  return __builtin_copysignf(r, d);
+}
+
+// vector_field::contains (vector_field.cpp:12-21) sharing the quotients
+// with interpolate: q_i = (p_i - o_i) / h_i; inside iff 0 <= floor(q_i) <=
+// dims_i - 2 (a negative or NaN subscript fails through the reference's
+// size_t cast).
+__device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, float pz, float d[3], float q[3]) {
+  d[0] = px - f.offset[0];
+  d[1] = py - f.offset[1];
+  d[2] = pz - f.offset[2];
+  bool inside = true;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    q[i] = d[i] / f.spacing[i];
+    const float s = floorf(q[i]);
+    inside = inside && (s >= 0.0f) && (s <= static_cast<float>(f.dims[i] - 2));
+  }
+  return inside;
+}
+
+__device__ __forceinline__ float lerp1(float a, float b, float w, float u) { return u * a + w * b; }
+
+// vector_field::interpolate (vector_field.cpp:22-48): m = floor(q), w =
+// fmod(d, h) / h, lerp (linear_interpolate.hpp:6-10) along z, y, x.  With
+// INTERP == PA_INTERP_CONSISTENT the weight is q - floor(q) (exact in FP32
+// for 0 <= q < 2^23).
+template <int S, int INTERP>
+__device__ __forceinline__ void interpolate(const FieldDesc& f, const float d[3], const float q[3], float v[3]) {
+  int m[3];
+  float w[3], u[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float fl = floorf(q[i]);
+    m[i] = static_cast<int>(fl);
+    if constexpr (INTERP == PA_INTERP_CONSISTENT) {
+      w[i] = q[i] - fl;
+    } else {
+      w[i] = exact_fmod(d[i], f.spacing[i], q[i]) / f.spacing[i];
+    }
+    u[i] = 1.0f - w[i];
+  }
+  const int64_t i000 = static_cast<int64_t>(m[0]) * f.stride0 + static_cast<int64_t>(m[1]) * f.stride1 + m[2];
+  const int64_t i010 = i000 + f.stride1;
+  const int64_t i100 = i000 + f.stride0;
+  const int64_t i110 = i100 + f.stride1;
+  float a0, a1, a2, b0, b1, b2, c0, c1, c2, e0, e1, e2;
+  float g0, g1, g2, h0, h1, h2, k0, k1, k2, l0, l1, l2;
+  load_node<S>(f.data, i000, a0, a1, a2);
+  load_node<S>(f.data, i000 + 1, b0, b1, b2);
+  load_node<S>(f.data, i010, c0, c1, c2);
+  load_node<S>(f.data, i010 + 1, e0, e1, e2);
+  load_node<S>(f.data, i100, g0, g1, g2);
+  load_node<S>(f.data, i100 + 1, h0, h1, h2);
+  load_node<S>(f.data, i110, k0, k1, k2);
+  load_node<S>(f.data, i110 + 1, l0, l1, l2);
+  // c00 = lerp(c000, c001, w2), c01 = lerp(c010, c011, w2), ...
+  const float c00x = lerp1(a0, b0, w[2], u[2]), c00y = lerp1(a1, b1, w[2], u[2]), c00z = lerp1(a2, b2, w[2], u[2]);
+  const float c01x = lerp1(c0, e0, w[2], u[2]), c01y = lerp1(c1, e1, w[2], u[2]), c01z = lerp1(c2, e2, w[2], u[2]);
+  const float c10x = lerp1(g0, h0, w[2], u[2]), c10y = lerp1(g1, h1, w[2], u[2]), c10z = lerp1(g2, h2, w[2], u[2]);
+  const float c11x = lerp1(k0, l0, w[2], u[2]), c11y = lerp1(k1, l1, w[2], u[2]), c11z = lerp1(k2, l2, w[2], u[2]);
+  const float c0x = lerp1(c00x, c01x, w[1], u[1]), c0y = lerp1(c00y, c01y, w[1], u[1]), c0z = lerp1(c00z, c01z, w[1], u[1]);
+  const float c1x = lerp1(c10x, c11x, w[1], u[1]), c1y = lerp1(c10y, c11y, w[1], u[1]), c1z = lerp1(c10z, c11z, w[1], u[1]);
+  v[0] = lerp1(c0x, c1x, w[0], u[0]);
+  v[1] = lerp1(c0y, c1y, w[0], u[0]);
+  v[2] = lerp1(c0z, c1z, w[0], u[0]);
+}
+
+// Stop test (particle_tracer.cpp:312): Eigen isZero() with the float
+// dummy precision 1e-5f, or the |v| < eps extension.
+template <int STOP>
+__device__ __forceinline__ bool stops(const float v[3], float eps2) {
+  if constexpr (STOP == PA_STOP_NORM) {
+    const float n2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    return n2 < eps2;
+  } else {
+    return fabsf(v[0]) <= 1e-5f && fabsf(v[1]) <= 1e-5f && fabsf(v[2]) <= 1e-5f;
+  }
+}
+
+// Frozen-derivative step (particle_tracer.cpp:315-324 with odeint's
+// runge_kutta4 final stage: 1*x + b0dt*k + b1dt*k + b2dt*k + b3dt*k, left to
+// right; euler: 1*x + dt*k).  w is stepped with derivative 0 like the
+// reference's vector4 state.
+template <int MODE>
+__device__ __forceinline__ float4 step_frozen(const TraceConfig& c, float4 p, const float v[3]) {
+  float4 o;
+  if constexpr (MODE == STEP_EULER) {
+    o.x = p.x + c.dt * v[0];
+    o.y = p.y + c.dt * v[1];
+    o.z = p.z + c.dt * v[2];
+    o.w = p.w + c.dt * 0.0f;
+  } else {
+    o.x = (((p.x + c.a0 * v[0]) + c.a1 * v[0]) + c.a2 * v[0]) + c.a3 * v[0];
+    o.y = (((p.y + c.a0 * v[1]) + c.a1 * v[1]) + c.a2 * v[1]) + c.a3 * v[1];
+    o.z = (((p.z + c.a0 * v[2]) + c.a1 * v[2]) + c.a2 * v[2]) + c.a3 * v[2];
+    o.w = (((p.w + c.a0 * 0.0f) + c.a1 * 0.0f) + c.a2 * 0.0f) + c.a3 * 0.0f;
+  }
+  return o;
+}
+
+// Textbook RK4 (extension): odeint's generic-RK stage sums with the field
+// evaluated at every stage point.  False if a stage point leaves the block.
+template <int S, int INTERP>
+__device__ __forceinline__ bool step_classic(const TraceConfig& c, const FieldDesc& f, float4 p, const float k1[3],
+                                             float4& out) {
+  float d[3], q[3], k2[3], k3[3], k4[3];
+  float x0 = p.x + c.half_dt * k1[0];
+  float x1 = p.y + c.half_dt * k1[1];
+  float x2 = p.z + c.half_dt * k1[2];
+  if (!locate(f, x0, x1, x2, d, q)) return false;
+  interpolate<S, INTERP>(f, d, q, k2);
+  x0 = (p.x + c.zero_dt * k1[0]) + c.half_dt * k2[0];
+  x1 = (p.y + c.zero_dt * k1[1]) + c.half_dt * k2[1];
+  x2 = (p.z + c.zero_dt * k1[2]) + c.half_dt * k2[2];
+  if (!locate(f, x0, x1, x2, d, q)) return false;
+  interpolate<S, INTERP>(f, d, q, k3);
+  x0 = ((p.x + c.zero_dt * k1[0]) + c.zero_dt * k2[0]) + c.one_dt * k3[0];
+  x1 = ((p.y + c.zero_dt * k1[1]) + c.zero_dt * k2[1]) + c.one_dt * k3[1];
+  x2 = ((p.z + c.zero_dt * k1[2]) + c.zero_dt * k2[2]) + c.one_dt * k3[2];
+  if (!locate(f, x0, x1, x2, d, q)) return false;
+  interpolate<S, INTERP>(f, d, q, k4);
+  out.x = (((p.x + c.a0 * k1[0]) + c.a1 * k2[0]) + c.a2 * k3[0]) + c.a3 * k4[0];
+  out.y = (((p.y + c.a0 * k1[1]) + c.a1 * k2[1]) + c.a2 * k3[1]) + c.a3 * k4[1];
+  out.z = (((p.z + c.a0 * k1[2]) + c.a1 * k2[2]) + c.a2 * k3[2]) + c.a3 * k4[2];
+  out.w = (((p.w + c.a0 * 0.0f) + c.a1 * 0.0f) + c.a2 * 0.0f) + c.a3 * 0.0f;
+  return true;
+}
+
+// Neighbour choice of particle_tracer.cpp:288-293: first of x<min, x>max,
+// y<min, y>max, z<min, z>max whose neighbour exists (mask bit).
+__device__ __forceinline__ int face_rule(const FieldDesc& f, float4 p, int mask) {
+  if (p.x < f.offset[0] && (mask & 1)) return 0;
+  if (p.x > f.upper[0] && (mask & 2)) return 1;
+  if (p.y < f.offset[1] && (mask & 4)) return 2;
+  if (p.y > f.upper[1] && (mask & 8)) return 3;
+  if (p.z < f.offset[2] && (mask & 16)) return 4;
+  if (p.z > f.upper[2] && (mask & 32)) return 5;
+  return -1;
+}
+
+__device__ __forceinline__ float4 termination_vertex() { return make_float4(-2.0f, -2.0f, -2.0f, 0.0f); }
+
+// One field evaluation plus one step from `last`.  Returns 0 = stepped
+// (next valid), 1 = left the block (contains failed, or a classic stage
+// point did), 2 = stop rule.
+template <int S, int MODE, int INTERP, int STOP>
+__device__ __forceinline__ int advance(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next) {
+  float d[3], q[3], v[3];
+  if (!locate(f, last.x, last.y, last.z, d, q)) return 1;
+  interpolate<S, INTERP>(f, d, q, v);
+  if (stops<STOP>(v, c.stop_eps2)) return 2;
+  if constexpr (MODE == STEP_CLASSIC) {
+    if (!step_classic<S, INTERP>(c, f, last, v, next)) return 1;
+  } else {
+    next = step_frozen<MODE>(c, last, v);
+  }
+  return 0;
+}
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+// Calls f(IC<storage>, IC<mode>, IC<interp>) with the run-time choices as
+// compile-time constants (one kernel instantiation per combination).
+template <class F>
+inline void dispatch(int storage, int mode, int interp, F&& f) {
+  auto with_i = [&](auto s, auto m) {
+    if (interp == PA_INTERP_CONSISTENT) f(s, m, IC<PA_INTERP_CONSISTENT>{});
+    else f(s, m, IC<PA_INTERP_REFERENCE>{});
+  };
+  auto with_m = [&](auto s) {
+    if (mode == STEP_EULER) with_i(s, IC<STEP_EULER>{});
+    else if (mode == STEP_CLASSIC) with_i(s, IC<STEP_CLASSIC>{});
+    else with_i(s, IC<STEP_RK4>{});
+  };
+  if (storage == PA_STORAGE_BF16) with_m(IC<PA_STORAGE_BF16>{});
+  else if (storage == PA_STORAGE_FP16) with_m(IC<PA_STORAGE_FP16>{});
+  else with_m(IC<PA_STORAGE_FP32>{});
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+}  // namespace pa_amd

@@ -1,0 +1,120 @@
+// Internal declarations shared by the HIP translation units of libpa_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pa_gpu.h"
+
+static_assert(sizeof(pa_particle) == 48, "pa_particle must match pa::particle (48 B)");
+static_assert(sizeof(pa_round_info) == 40, "pa_round_info layout");
+
+namespace pa_amd {
+
+// Device-side description of one block.  upper = offset + size is computed
+// once on the host in FP32, the same single rounding the reference performs
+// at particle_tracer.cpp:263.
+struct FieldDesc {
+  const void* data;   // nodes, storage-dependent (4 components each)
+  int32_t dims[3];    // ghosted node counts
+  int32_t storage;    // pa_storage
+  float offset[3];
+  float spacing[3];
+  float upper[3];
+  float size[3];
+  int64_t stride0;    // dims[1] * dims[2]
+  int64_t stride1;    // dims[2]
+};
+
+// Per-launch tracer configuration.  Slot 0 is the own block (vector field
+// index -1), slots 1..6 the neighbours 0..5.
+struct TraceConfig {
+  FieldDesc field[7];
+  int32_t has_field[7];
+  int32_t local_mask;        // bit k: own block has a neighbour across face k
+  int32_t face_mask[6];      // bit d: neighbour block k has a neighbour across face d
+  float dt;
+  float a0, a1, a2, a3;      // frozen RK4 coefficients b_k * dt
+  float half_dt;             // (1/2) * dt for classic RK4 stage points
+  float zero_dt;             // 0 * dt (odeint's zero tableau entries)
+  float one_dt;              // 1 * dt
+  float stop_eps2;
+  int32_t interp;
+};
+
+}  // namespace pa_amd
+
+struct pa_gpu_field {
+  pa_amd::FieldDesc desc;
+  void* d_nodes = nullptr;
+  size_t bytes = 0;
+};
+
+struct pa_gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  // Kernel timing
+  bool timing = false;
+  struct timed {
+    hipEvent_t a, b;
+    std::string name;
+  };
+  std::vector<timed> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, std::pair<double, uint64_t>> kernel_ms;
+  // Scratch that grows on demand (block sums, counters, flags).
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void* counters = nullptr;  // 4 KiB of device counters
+  void* pinned = nullptr;    // 4 KiB of pinned host memory for counter readback
+};
+
+struct pa_gpu_tracer {
+  pa_gpu_ctx* ctx = nullptr;
+  const pa_gpu_field* local = nullptr;
+  const pa_gpu_field* neighbors[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int32_t neighbor_rank[6] = {-1, -1, -1, -1, -1, -1};
+  int32_t face_mask[6] = {0, 0, 0, 0, 0, 0};
+  int32_t integrator = PA_INTEGRATOR_EULER;  // the reference default (particle_tracer.hpp:68)
+  float step_size = 1.0f;                    // particle_tracer.hpp:69
+  int32_t stop_rule = PA_STOP_ISZERO;
+  float stop_eps = 0.0f;
+  int32_t interp = PA_INTERP_REFERENCE;
+};
+
+namespace pa_amd {
+
+pa_status fail(pa_gpu_ctx* ctx, pa_status s, const std::string& msg);
+pa_status check_hip(pa_gpu_ctx* ctx, hipError_t e, const char* what);
+// Grow-only scratch on the context.
+pa_status scratch(pa_gpu_ctx* ctx, size_t bytes, void** out);
+// Timing brackets around a launch.
+void launch_begin(pa_gpu_ctx* ctx, const char* name);
+void launch_end(pa_gpu_ctx* ctx);
+// Build the per-launch configuration from a tracer.
+pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg);
+
+// Exclusive scan of n int64 values in place on the device (n+1 outputs:
+// out[n] = total).  Uses ctx scratch.
+pa_status exclusive_scan_i64(pa_gpu_ctx* ctx, const int64_t* d_in, int64_t* d_out, size_t n);
+
+}  // namespace pa_amd
+
+#define PA_HIP(ctx, expr)                                           \
+  do {                                                              \
+    hipError_t e__ = (expr);                                        \
+    if (e__ != hipSuccess) return pa_amd::check_hip(ctx, e__, #expr); \
+  } while (0)
+
+#define PA_TRY(expr)                 \
+  do {                               \
+    pa_status s__ = (expr);          \
+    if (s__ != PA_OK) return s__;    \
+  } while (0)

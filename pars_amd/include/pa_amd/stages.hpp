@@ -1,0 +1,322 @@
+// pa_amd/stages.hpp — C++ mirror of the reference's `pa` stage classes
+// (acdemiralp/pars, pa/include/pa/stages/*.hpp) over the C ABI of
+// libpa_gpu.so (include/pa_gpu.h).  Class and method names, argument order
+// and meaning follow the reference so a pipeline written against `pa::`
+// switches by namespace; host std::vectors are copied to and from HBM at
+// the boundary exactly where the reference hands them between stages.
+// Errors throw pa_amd::error (the reference has no status codes; its
+// failures are exceptions from allocation or crashes).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pa_gpu.h"
+
+namespace pa_amd {
+
+using scalar = float;
+using integer = std::int32_t;
+using particle = pa_particle;  // pa::particle (particle.hpp:30-35)
+struct vector3 { float x, y, z; };
+struct vector4 { float x, y, z, w; };
+static_assert(sizeof(vector4) == 16, "vector4 is 16 bytes");
+
+// integral_curves.hpp:11-20
+struct integral_curves {
+  std::vector<vector4> vertices;
+  std::vector<vector4> colors;
+  std::vector<integer> indices;
+};
+inline constexpr vector4 termination_vertex{-2.0f, -2.0f, -2.0f, 0.0f};
+inline constexpr vector4 invalid_vertex{-1.0f, -1.0f, -1.0f, 0.0f};
+inline constexpr integer invalid_index = -1;
+
+struct error : std::runtime_error {
+  pa_status status;
+  error(pa_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+class context {
+ public:
+  explicit context(int device = 0) {
+    pa_status s = pa_gpu_ctx_create(device, &ctx_);
+    if (s != PA_OK) throw error(s, "pa_gpu_ctx_create failed");
+  }
+  ~context() {
+    if (ctx_) pa_gpu_ctx_destroy(ctx_);
+  }
+  context(const context&) = delete;
+  context& operator=(const context&) = delete;
+  pa_gpu_ctx* get() const { return ctx_; }
+  void check(pa_status s, const char* what) const {
+    if (s != PA_OK) throw error(s, std::string(what) + ": " + pa_gpu_last_error(ctx_));
+  }
+
+ private:
+  pa_gpu_ctx* ctx_ = nullptr;
+};
+
+// Device buffer of T (RAII over pa_gpu_malloc / pa_gpu_free).
+template <class T>
+class device_vector {
+ public:
+  device_vector(context& c, size_t n) : c_(&c), n_(n) {
+    void* p = nullptr;
+    c.check(pa_gpu_malloc(c.get(), (n ? n : 1) * sizeof(T), &p), "pa_gpu_malloc");
+    p_ = static_cast<T*>(p);
+  }
+  device_vector(context& c, const std::vector<T>& h) : device_vector(c, h.size()) {
+    if (!h.empty()) c.check(pa_gpu_memcpy_h2d(c.get(), p_, h.data(), h.size() * sizeof(T)), "h2d");
+  }
+  ~device_vector() {
+    if (p_) pa_gpu_free(c_->get(), p_);
+  }
+  device_vector(const device_vector&) = delete;
+  device_vector& operator=(const device_vector&) = delete;
+  T* data() const { return p_; }
+  size_t size() const { return n_; }
+  std::vector<T> to_host(size_t count) const {
+    std::vector<T> h(count);
+    if (count) c_->check(pa_gpu_memcpy_d2h(c_->get(), h.data(), p_, count * sizeof(T)), "d2h");
+    return h;
+  }
+
+ private:
+  context* c_;
+  size_t n_;
+  T* p_ = nullptr;
+};
+
+// pa::vector_field (vector_field.hpp:14-24), resident in HBM.
+class vector_field {
+ public:
+  vector_field(context& c, const std::vector<vector3>& data, const std::array<int32_t, 3>& dims, vector3 offset,
+               vector3 size, vector3 spacing, pa_storage storage = PA_STORAGE_FP32)
+      : offset(offset), size(size), spacing(spacing), dims(dims), c_(&c) {
+    const float o[3] = {offset.x, offset.y, offset.z}, s[3] = {size.x, size.y, size.z},
+                h[3] = {spacing.x, spacing.y, spacing.z};
+    c.check(pa_gpu_field_create(c.get(), &data[0].x, dims.data(), o, s, h, storage, &f_), "field_create");
+  }
+  ~vector_field() {
+    if (f_) pa_gpu_field_destroy(c_->get(), f_);
+  }
+  vector_field(const vector_field&) = delete;
+  vector_field& operator=(const vector_field&) = delete;
+  const pa_gpu_field* get() const { return f_; }
+
+  vector3 offset, size, spacing;
+  std::array<int32_t, 3> dims;
+
+ private:
+  context* c_;
+  pa_gpu_field* f_ = nullptr;
+};
+
+// pa::seed_generator (seed_generator.hpp:10-14).
+class seed_generator {
+ public:
+  static std::vector<particle> generate(context& c, const vector3& offset, const vector3& size, const vector3& stride,
+                                        integer remaining_iterations, integer rank) {
+    const float o[3] = {offset.x, offset.y, offset.z}, s[3] = {size.x, size.y, size.z},
+                t[3] = {stride.x, stride.y, stride.z};
+    int32_t dims[3];
+    size_t n = 0;
+    c.check(pa_gpu_seed_count(s, t, dims, &n), "seed_count");
+    device_vector<particle> d(c, n);
+    size_t got = 0;
+    c.check(pa_gpu_seed_generate(c.get(), o, s, t, remaining_iterations, rank, d.data(), n, &got), "seed_generate");
+    return d.to_host(got);
+  }
+};
+
+// pa::particle_tracer (particle_tracer.hpp:23-71) for one rank's round.
+class particle_tracer {
+ public:
+  struct round_info : pa_round_info {
+    std::array<std::vector<particle>, 6> out_of_bounds_particles;           // by neighbour slot
+    std::array<std::vector<particle>, 6> neighbor_out_of_bounds_particles;  // by neighbour slot
+  };
+
+  explicit particle_tracer(context& c) : c_(&c) { c.check(pa_gpu_tracer_create(c.get(), &t_), "tracer_create"); }
+  ~particle_tracer() {
+    if (t_) pa_gpu_tracer_destroy(t_);
+  }
+  particle_tracer(const particle_tracer&) = delete;
+  particle_tracer& operator=(const particle_tracer&) = delete;
+
+  void set_local_vector_field(const vector_field* f) {
+    c_->check(pa_gpu_tracer_set_local_vector_field(t_, f ? f->get() : nullptr), "set_local_vector_field");
+  }
+  void set_neighbor_vector_fields(const std::array<const vector_field*, 6>& f, const std::array<int32_t, 6>& ranks) {
+    const pa_gpu_field* h[6];
+    for (int k = 0; k < 6; ++k) h[k] = f[k] ? f[k]->get() : nullptr;
+    c_->check(pa_gpu_tracer_set_neighbor_vector_fields(t_, h, ranks.data()), "set_neighbor_vector_fields");
+  }
+  void set_integrator(pa_integrator i) { c_->check(pa_gpu_tracer_set_integrator(t_, i), "set_integrator"); }
+  void set_step_size(scalar s) { c_->check(pa_gpu_tracer_set_step_size(t_, s), "set_step_size"); }
+
+  // compute_round_info + allocate + initialize + trace + prune of one round
+  // (particle_tracer.cpp:210-337, 402-409): returns the pruned curves of the
+  // round and fills the hand-off maps of `info`.
+  std::vector<integral_curves> trace(const std::vector<particle>& particles, round_info* info = nullptr) {
+    std::vector<integral_curves> curves;
+    if (particles.empty()) return curves;
+    device_vector<particle> d(*c_, particles);
+    round_info ri;
+    c_->check(pa_gpu_tracer_compute_round_info(t_, d.data(), particles.size(), 0, &ri), "compute_round_info");
+    const size_t maxr = ri.maximum_remaining_iterations;
+    const size_t slots = particles.size() * maxr;
+    device_vector<vector4> chunk(*c_, slots);
+    std::vector<float*> table;
+    for (uint64_t k = 0; k < ri.integral_curve_count; ++k)
+      table.push_back(reinterpret_cast<float*>(chunk.data() + k * ri.vertices_per_integral_curve));
+    device_vector<int32_t> counts(*c_, particles.size());
+    device_vector<particle> hand(*c_, particles.size());
+    pa_handoff h{hand.data(), nullptr, particles.size(), {{0}}};
+    uint64_t steps = 0;
+    c_->check(pa_gpu_tracer_trace(t_, d.data(), particles.size(), &ri, table.data(), counts.data(), &h, &steps),
+              "trace");
+    // prune each chunk by the per-particle counts
+    const size_t ppic = ri.vertices_per_integral_curve / maxr;
+    for (uint64_t k = 0; k < ri.integral_curve_count; ++k) {
+      const size_t first = k * ppic;
+      const size_t np = first >= particles.size() ? 0 : std::min(ppic, particles.size() - first);
+      integral_curves ic;
+      if (np) {
+        device_vector<vector4> out(*c_, np * maxr);
+        size_t m = 0;
+        c_->check(pa_gpu_prune_counted(c_->get(), reinterpret_cast<float*>(chunk.data() + first * maxr), maxr,
+                                       counts.data() + first, np, reinterpret_cast<float*>(out.data()), np * maxr, &m,
+                                       nullptr),
+                  "prune");
+        ic.vertices = out.to_host(m);
+      }
+      curves.push_back(std::move(ic));
+    }
+    if (info) {
+      static_cast<pa_round_info&>(*info) = ri;
+      std::vector<particle> all = hand.to_host(particles.size());
+      size_t at = 0;
+      for (int kind = 0; kind < 2; ++kind)
+        for (int s = 0; s < 6; ++s) {
+          auto& dst = kind == 0 ? info->out_of_bounds_particles[s] : info->neighbor_out_of_bounds_particles[s];
+          dst.assign(all.begin() + at, all.begin() + at + h.counts[kind][s]);
+          at += h.counts[kind][s];
+        }
+    }
+    return curves;
+  }
+
+ private:
+  context* c_;
+  pa_gpu_tracer* t_ = nullptr;
+};
+
+// pa::index_generator (index_generator.hpp:10-14).
+class index_generator {
+ public:
+  static void generate(context& c, integral_curves* curves) {
+    if (curves->vertices.empty()) throw error(PA_E_INVALID, "index generation of an empty vertex array");
+    device_vector<vector4> v(c, curves->vertices);
+    device_vector<int32_t> idx(c, curves->vertices.size());
+    size_t m = 0;
+    c.check(pa_gpu_index_generate(c.get(), reinterpret_cast<const float*>(v.data()), curves->vertices.size(),
+                                  idx.data(), &m),
+            "index_generate");
+    curves->indices = idx.to_host(m);
+  }
+};
+
+// pa::color_generator (color_generator.hpp:10-32).
+class color_generator {
+ public:
+  static void generate(context& c, integral_curves* curves, pa_color_mode mode = PA_COLOR_RGB,
+                       scalar constant_parameter = 0.5f) {
+    const size_t n = curves->vertices.size();
+    curves->colors.assign(n, vector4{0.0f, 0.0f, 0.0f, 0.0f});
+    if (curves->indices.empty()) return;
+    device_vector<vector4> v(c, curves->vertices);
+    device_vector<int32_t> idx(c, curves->indices);
+    device_vector<vector4> col(c, curves->colors);
+    c.check(pa_gpu_color_generate(c.get(), reinterpret_cast<const float*>(v.data()), n, idx.data(),
+                                  curves->indices.size(), mode, constant_parameter,
+                                  reinterpret_cast<float*>(col.data())),
+            "color_generate");
+    curves->colors = col.to_host(n);
+  }
+};
+
+// pa::flow_map_generator (flow_map_generator.hpp:12-28) + ftle_map_generator
+// (ftle_map_generator.hpp:13-40) for one block without neighbours.
+class flow_map_generator {
+ public:
+  explicit flow_map_generator(context& c) : c_(&c) { c.check(pa_gpu_tracer_create(c.get(), &t_), "tracer_create"); }
+  ~flow_map_generator() {
+    if (t_) pa_gpu_tracer_destroy(t_);
+  }
+  flow_map_generator(const flow_map_generator&) = delete;
+  flow_map_generator& operator=(const flow_map_generator&) = delete;
+  void set_vector_field(const vector_field* f) {
+    field_ = f;
+    c_->check(pa_gpu_tracer_set_local_vector_field(t_, f->get()), "set_vector_field");
+  }
+  void set_integrator(pa_integrator i) { c_->check(pa_gpu_tracer_set_integrator(t_, i), "set_integrator"); }
+  void set_step_size(scalar s) { c_->check(pa_gpu_tracer_set_step_size(t_, s), "set_step_size"); }
+  // Returns the flow map [d0][d1][d2] xyz on the host.
+  std::vector<vector3> generate(const std::array<int32_t, 3>& block_size, size_t iterations,
+                                scalar resolution_scale, std::array<int32_t, 3>* dims) {
+    std::array<int32_t, 3> d;
+    for (int i = 0; i < 3; ++i) d[i] = int32_t(float(block_size[i]) * resolution_scale);
+    const size_t n = size_t(d[0]) * size_t(d[1]) * size_t(d[2]);
+    device_vector<vector3> fm(*c_, n);
+    uint64_t steps = 0;
+    c_->check(pa_gpu_flow_map_generate(t_, block_size.data(), iterations, resolution_scale, 0,
+                                       reinterpret_cast<float*>(fm.data()), d.data(), &steps),
+              "flow_map_generate");
+    if (dims) *dims = d;
+    return fm.to_host(n);
+  }
+
+ private:
+  context* c_;
+  pa_gpu_tracer* t_ = nullptr;
+  const vector_field* field_ = nullptr;
+};
+
+class ftle_map_generator {
+ public:
+  enum class mode { regular = PA_FTLE_REGULAR, fractional_anisotropy = PA_FTLE_FRACTIONAL_ANISOTROPY };
+  explicit ftle_map_generator(context& c, mode m = mode::regular) : c_(&c), mode_(m) {}
+  void set_flow_map(const std::vector<vector3>* flow_map, const std::array<int32_t, 3>& dims, vector3 spacing) {
+    flow_map_ = flow_map;
+    dims_ = dims;
+    spacing_ = spacing;
+  }
+  void set_time(scalar t) { time_ = t; }
+  std::vector<float> generate() {
+    const size_t n = flow_map_->size();
+    device_vector<vector3> fm(*c_, *flow_map_);
+    device_vector<float> out(*c_, n);
+    const float h[3] = {spacing_.x, spacing_.y, spacing_.z};
+    c_->check(pa_gpu_ftle_generate(c_->get(), reinterpret_cast<const float*>(fm.data()), dims_.data(), h, time_,
+                                   pa_ftle_mode(mode_), out.data(), nullptr),
+              "ftle_generate");
+    return out.to_host(n);
+  }
+
+ private:
+  context* c_;
+  mode mode_;
+  const std::vector<vector3>* flow_map_ = nullptr;
+  std::array<int32_t, 3> dims_{};
+  vector3 spacing_{};
+  scalar time_ = 1.0f;
+};
+
+}  // namespace pa_amd

@@ -1,0 +1,68 @@
+// Drives one streamline pass through the C++ stage mirror (pa_amd/stages.hpp)
+// the way pars/source/pipeline.cpp:107-260 drives pa::: seeds over the own
+// block, one trace round, prune, index, colour.  Used by
+// tests/test_gpu_cpp_mirror.py, which compares the files it writes against
+// the oracle.
+//
+// usage: stages_demo <field.f32> <n> <R> <dt> <out_prefix>
+//   field.f32: n^3 x 3 float32 (C order, z fastest), spacing 2pi/(n-1)
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <vector>
+
+#include "pa_amd/stages.hpp"
+
+template <class T>
+static void dump(const std::string& path, const std::vector<T>& v) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), std::streamsize(v.size() * sizeof(T)));
+}
+
+int main(int argc, char** argv) {
+  if (argc != 6) {
+    std::fprintf(stderr, "usage: %s field.f32 n R dt out_prefix\n", argv[0]);
+    return 2;
+  }
+  const int n = std::atoi(argv[2]);
+  const int R = std::atoi(argv[3]);
+  const float dt = float(std::atof(argv[4]));
+  const std::string out = argv[5];
+  std::vector<pa_amd::vector3> data(size_t(n) * n * n);
+  {
+    std::ifstream f(argv[1], std::ios::binary);
+    f.read(reinterpret_cast<char*>(data.data()), std::streamsize(data.size() * sizeof(pa_amd::vector3)));
+    if (!f) {
+      std::fprintf(stderr, "cannot read %s\n", argv[1]);
+      return 2;
+    }
+  }
+  try {
+    pa_amd::context ctx(0);
+    const float h = float(2.0 * 3.14159265358979323846 / (n - 1));
+    const float s = float(n) * h;
+    pa_amd::vector_field field(ctx, data, {n, n, n}, {0, 0, 0}, {s, s, s}, {h, h, h});
+    const float stride = 3.0f * h;
+    auto seeds = pa_amd::seed_generator::generate(ctx, {0, 0, 0}, {s, s, s}, {stride, stride, stride}, R, 0);
+    pa_amd::particle_tracer tracer(ctx);
+    tracer.set_local_vector_field(&field);
+    tracer.set_integrator(PA_INTEGRATOR_RUNGE_KUTTA_4);
+    tracer.set_step_size(dt);
+    pa_amd::particle_tracer::round_info info;
+    auto curves = tracer.trace(seeds, &info);
+    auto& c = curves.at(0);
+    pa_amd::index_generator::generate(ctx, &c);
+    pa_amd::color_generator::generate(ctx, &c, PA_COLOR_HSV_CONSTANT_S, 0.75f);
+    dump(out + ".seeds", seeds);
+    dump(out + ".vertices", c.vertices);
+    dump(out + ".indices", c.indices);
+    dump(out + ".colors", c.colors);
+    std::printf("stages_demo: %zu seeds, %zu vertices, %zu indices\n", seeds.size(), c.vertices.size(),
+                c.indices.size());
+  } catch (const pa_amd::error& e) {
+    std::fprintf(stderr, "pa_amd error %d: %s\n", int(e.status), e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,40 @@
+"""The C++ stage mirror (pars_amd/include/pa_amd/stages.hpp) drives the
+GPU through the C ABI like pars' pipeline drives pa::; its outputs match the
+oracle's seed -> trace -> prune -> index -> colour chain."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from pars_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(os.path.dirname(__file__), "cpp", "stages_demo")
+
+
+def test_cpp_mirror_pipeline(orc, tmp_path):
+    assert os.path.exists(EXE), "built by __graft_entry__.build()"
+    n, R, dt = 40, 150, 0.01
+    h = 2 * np.pi / (n - 1)
+    u = W.abc_block_f64(W.BlockInfo(0, (0, 0, 0), (0, 0, 0), (n, n, n)), h).astype(np.float32)
+    fin = tmp_path / "field.f32"
+    u.tofile(fin)
+    out = str(tmp_path / "run")
+    r = subprocess.run([EXE, str(fin), str(n), str(R), repr(dt), out], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    hf = np.float32(h)
+    size = [np.float32(n) * hf] * 3
+    seeds = orc.seed_generate((0, 0, 0), size, [np.float32(3.0) * hf] * 3, R, 0)
+    got_seeds = np.fromfile(out + ".seeds", orc.PARTICLE_DTYPE)
+    assert got_seeds.tobytes() == seeds.tobytes()
+    F = orc.Field(u, (0, 0, 0), size, [hf] * 3)
+    _, chunks, _, _, _ = orc.trace(seeds, F, dt=np.float32(dt), integrator=orc.RK4)
+    v = orc.prune(chunks[0])
+    got_v = np.fromfile(out + ".vertices", np.float32).reshape(-1, 4)
+    assert got_v.tobytes() == v.tobytes()
+    idx = orc.index_generate(v)
+    assert np.array_equal(np.fromfile(out + ".indices", np.int32), idx)
+    col = orc.color_generate(v, idx, 2, 0.75)
+    np.testing.assert_allclose(np.fromfile(out + ".colors", np.float32).reshape(-1, 4), col, atol=1e-6)

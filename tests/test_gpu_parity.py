@@ -1,0 +1,289 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same
+seeded inputs.  Integer, index and slot results are bit-exact; FP32 vertices
+are compared bit-exactly as well (the GPU arithmetic is IEEE-identical to the
+reference's), which implies the 1e-5 relative bar; colours from glibc vs
+device atan2f/acosf are compared at 1e-6 absolute."""
+import numpy as np
+import pytest
+
+import pars_amd as pa
+from pars_amd import workloads as W
+
+from .helpers import ShareFields, f32, oracle_integrator, single_block
+
+pytestmark = pytest.mark.gpu
+
+STORAGES = [pa.FP32, pa.BF16, pa.FP16]
+
+
+def rel_ok(a, b, tol=1e-5):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.all(np.abs(a - b) <= tol * np.maximum(np.abs(b), 1.0))
+
+
+def test_seeds_bitexact(ctx, orc):
+    cases = []
+    h = f32(2 * np.pi / 63)
+    cases.append(((0, 0, 0), [f32(64) * h] * 3, [f32(4) * h] * 3, 258, 0))
+    h = f32(2 * np.pi / 511)
+    o = f32(128) * h
+    cases.append(((o, o, o), [f32(128) * h] * 3, [f32(2) * h] * 3, 1002, 21))
+    cases.append(((0.5, -1.0, 2.0), (1.0, 2.0, 3.0), (0.3, 0.25, 0.5), 5, 3))
+    for off, size, stride, R, rank in cases:
+        d = pa.seed_generator.generate(ctx, off, size, stride, R, rank).download()
+        o = orc.seed_generate(off, size, stride, R, rank)
+        assert len(d) == len(o)
+        assert d.tobytes() == o.tobytes()
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+def test_field_storage_roundtrip(ctx, storage):
+    rng = np.random.default_rng(11)
+    data = (rng.normal(size=(9, 8, 7, 3)) * 3).astype(np.float32)
+    data[0, 0, 0] = [1e-30, -0.0, 65504.0]
+    f = pa.VectorField.from_host(ctx, data, (0, 0, 0), (8, 7, 6), (1, 1, 1), storage)
+    got = f.read()
+    want = W.quantize(data, storage)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert f.nbytes() == data.shape[0] * data.shape[1] * data.shape[2] * (16 if storage == pa.FP32 else 8)
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+@pytest.mark.parametrize("interp", [pa.INTERP_REFERENCE, pa.INTERP_CONSISTENT])
+def test_interpolate_bitexact(ctx, orc, storage, interp):
+    d, o, _ = single_block(ctx, orc, 33, storage)
+    rng = np.random.default_rng(12)
+    h = o.spacing[0]
+    n = 20000
+    pts = np.zeros((n, 4), np.float32)
+    pts[:, :3] = rng.uniform(-0.1, f32(33) * h + 0.1, size=(n, 3)).astype(np.float32)
+    # node planes (where the reference's rounded quotient and exact remainder disagree)
+    k = rng.integers(0, 33, size=(4000, 3))
+    pts[:4000, :3] = k.astype(np.float32) * h
+    pts[4000:4100, 0] = -0.0
+    pts[4100:4110, 1] = np.nan
+    got = d.interpolate(pts, interp)
+    for i in range(n):
+        inside = orc.contains(o, pts[i])
+        assert got[i, 3] == (1.0 if inside else 0.0), i
+        if inside:
+            want = orc.interpolate(o, pts[i], interp)
+            assert got[i, :3].tobytes() == want.tobytes(), (i, pts[i], got[i], want)
+
+
+def _trace_both(ctx, orc, seeds_np, dev_tracer, orc_kwargs, local_orc):
+    parts = pa.Particles.from_host(ctx, seeds_np)
+    info = dev_tracer.compute_round_info(parts)
+    oinfo = orc.round_info(seeds_np)
+    assert (info.maximum_remaining_iterations, info.vertices_per_integral_curve,
+            info.vertices_per_integral_curve_last, info.integral_curve_offset,
+            info.integral_curve_count) == oinfo
+    chunks = dev_tracer.allocate(info, fill_invalid=True)
+    res = dev_tracer.trace(parts, chunks, info)
+    o_info, o_chunks, o_counts, o_hand, o_steps = orc.trace(seeds_np, local_orc, **orc_kwargs)
+    return chunks, res, o_chunks, o_counts, o_hand, o_steps
+
+
+def test_trace_config1_bitexact(ctx, orc):
+    """ABC 64^3, 4096 seeds (stride 4), R = 258, dt = 0.01, runge_kutta_4."""
+    d, o, _ = single_block(ctx, orc, 64)
+    h = o.spacing[0]
+    seeds = orc.seed_generate((0, 0, 0), o.size, [f32(4) * h] * 3, 258, 0)
+    t = pa.particle_tracer(ctx)
+    t.set_local_vector_field(d)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.01)
+    chunks, res, oc, ocnt, ohand, osteps = _trace_both(ctx, orc, seeds, t, dict(dt=0.01, integrator=orc.RK4), o)
+    got = chunks[0][0].download(np.float32, chunks[0][1] * 4).reshape(-1, 4)
+    assert np.array_equal(res["counts"].download(np.int32, len(seeds)), ocnt)
+    assert res["total_steps"] == osteps
+    assert got.tobytes() == oc[0].tobytes()
+    assert rel_ok(got, oc[0])
+    assert res["handoff_counts"].sum() == 0 and len(ohand) == 0
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+@pytest.mark.parametrize("integ", [pa.RUNGE_KUTTA_4, pa.EULER, pa.CLASSIC_RK4])
+@pytest.mark.parametrize("interp", [pa.INTERP_REFERENCE, pa.INTERP_CONSISTENT])
+def test_trace_share_bitexact(ctx, orc, storage, integ, interp):
+    """One rank's share (own block + 6 neighbours, 4x4x4 grid of a 64^3 ABC
+    domain, rank 21), own-block seeds plus load-balanced particles that live in
+    neighbour blocks; slots, counts and both hand-off maps compared."""
+    S = ShareFields(ctx, orc, 64, 64, 21, storage)
+    sh = S.share
+    dt = 0.02
+    t = S.tracer(ctx, dt, integ, interp)
+    own = orc.seed_generate(sh.local.offset, sh.local.size, sh.local.spacing * f32(2), 150, 21)
+    extra = []
+    for k, nb in enumerate(sh.neighbors):
+        p = orc.seed_generate(nb.offset, nb.size, nb.spacing * f32(4), 120, nb.rank)
+        p["vector_field_index"] = k
+        extra.append(p)
+    seeds = np.concatenate([own] + extra)
+    oi = oracle_integrator(orc, integ)
+    chunks, res, oc, ocnt, ohand, osteps = _trace_both(ctx, orc, seeds, t, S.oracle_kwargs(dt, oi, interp), S.orc[0])
+    got = chunks[0][0].download(np.float32, chunks[0][1] * 4).reshape(-1, 4)
+    assert np.array_equal(res["counts"].download(np.int32, len(seeds)), ocnt)
+    assert res["total_steps"] == osteps and osteps > 0
+    assert got.tobytes() == oc[0].tobytes()
+    hc = res["handoff_counts"]
+    want_hc = np.zeros((2, 6), np.int64)
+    for h in ohand:
+        want_hc[h["kind"], h["slot"]] += 1
+    assert np.array_equal(hc, want_hc) and hc[0].sum() > 0 and hc[1].sum() > 0
+    assert res["handoff"].tobytes() == ohand["particle"].tobytes()
+    assert np.array_equal(res["handoff_sources"], ohand["source"])
+
+
+@pytest.mark.parametrize("storage", [pa.FP32, pa.BF16])
+@pytest.mark.parametrize("region_extra", [64, 1])
+def test_trace_union_bitexact(ctx, orc, storage, region_extra):
+    """Union-of-7 continuation vs the oracle's round emulation: every seed's
+    segments, in order; a tiny region forces the continuation path."""
+    S = ShareFields(ctx, orc, 64, 64, 21, storage)
+    sh = S.share
+    dt, R = 0.02, 400
+    t = S.tracer(ctx, dt)
+    seeds = orc.seed_generate(sh.local.offset, sh.local.size, sh.local.spacing * f32(2), R, 21)
+    ou = orc.trace_union(seeds, S.orc[0], S.orc[1:], sh.neighbor_ranks, sh.face_masks, dt=dt, integrator=orc.RK4)
+    region = R + region_extra
+    n = len(seeds)
+    pending = pa.Particles.from_host(ctx, seeds)
+    seed_of = np.arange(n)
+    got = {i: [] for i in range(n)}
+    steps = 0
+    for _ in range(64):
+        r = t.trace_union(pending, region)
+        steps += r["total_steps"]
+        cnt = r["counts"].download(np.int32, pending.n)
+        v = r["vertices"].download(np.float32, pending.n * region * 4).reshape(pending.n, region, 4)
+        for j in range(pending.n):
+            got[int(seed_of[j])].append(v[j, :cnt[j]])
+        cont = r["continuation"]
+        if cont.n == 0:
+            break
+        cp = cont.download()
+        order = np.argsort(cp["original_voxel"][:, 0], kind="stable")
+        cp = cp[order]
+        seed_of = seed_of[cp["original_voxel"][:, 0]]
+        pending = pa.Particles.from_host(ctx, cp)
+    else:
+        pytest.fail("continuation did not drain")
+    assert steps == ou["total_steps"]
+    for i in range(n):
+        sel = np.nonzero(ou["segment_seed"] == i)[0]
+        want = np.concatenate([ou["vertices"][ou["segment_begin"][s]:ou["segment_begin"][s] + ou["segment_length"][s]]
+                               for s in sel])
+        mine = np.concatenate(got[i])
+        assert mine.tobytes() == want.tobytes(), i
+    assert (ou["segment_block"] >= 0).any()  # some seeds did cross into neighbours
+
+
+def test_prune_index_color_parity(ctx, orc):
+    d, o, _ = single_block(ctx, orc, 48)
+    h = o.spacing[0]
+    seeds = orc.seed_generate((0, 0, 0), o.size, [f32(3) * h] * 3, 120, 0)
+    t = pa.particle_tracer(ctx)
+    t.set_local_vector_field(d)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.02)
+    chunks, res, oc, ocnt, _, _ = _trace_both(ctx, orc, seeds, t, dict(dt=0.02, integrator=orc.RK4), o)
+    buf, nslots = chunks[0]
+    pruned, m = pa.particle_tracer.prune(ctx, buf, nslots)
+    opr = orc.prune(oc[0])
+    assert m == len(opr)
+    assert pruned.download(np.float32, 4 * m).tobytes() == opr.tobytes()
+    # counted prune of the same round equals the value prune
+    packed, m2, offs = pa.particle_tracer.prune_counted(ctx, buf.ptr, 120, res["counts"], len(seeds))
+    assert m2 == m and packed.download(np.float32, 4 * m).tobytes() == opr.tobytes()
+    idx, ni = pa.index_generator.generate(ctx, pruned, m)
+    oidx = orc.index_generate(opr)
+    assert ni == len(oidx) and np.array_equal(idx.download(np.int32, ni), oidx)
+    for mode in (0, 1, 2, 3, 4, 100):
+        col = pa.color_generator.generate(ctx, pruned, m, idx, ni, mode, 0.6)
+        got = col.download(np.float32, 4 * m).reshape(-1, 4)
+        want = orc.color_generate(opr, oidx, mode, 0.6)
+        if mode == 4:
+            assert got.tobytes() == want.tobytes()
+        else:
+            np.testing.assert_allclose(got, want, atol=1e-6, rtol=0)
+
+
+@pytest.mark.parametrize("storage", STORAGES)
+def test_flow_map_and_ftle(ctx, orc, storage):
+    n, R, dt = 40, 101, 0.01
+    d, o, _ = single_block(ctx, orc, n, storage)
+    fmg = pa.flow_map_generator(ctx)
+    fmg.set_vector_field(d)
+    fmg.set_integrator("runge_kutta_4")
+    fmg.set_step_size(dt)
+    fm, dims, spacing, steps = fmg.generate((n, n, n), R, 1.0)
+    ofm, osteps = orc.flow_map(o, (n, n, n), R, 1.0, integrator=orc.RK4, dt=dt)
+    got = fm.download(np.float32, 3 * n ** 3).reshape(n, n, n, 3)
+    assert steps == osteps
+    assert got.tobytes() == ofm.tobytes()
+    T = (R - 1) * dt
+    for mode in (0, 1):
+        ftg = pa.ftle_map_generator(ctx, mode)
+        ftg.set_flow_map(fm, dims, spacing)
+        ftg.set_time(T)
+        out, lm = ftg.generate(want_lambda=True)
+        g = out.download(np.float32, n ** 3).reshape(n, n, n)
+        w, wl = orc.ftle(ofm, spacing, T, mode)
+        assert np.all(np.abs(g.astype(np.float64) - w) <= 1e-4 * np.maximum(np.abs(w), 1e-3))
+        glm = lm.download(np.float64, n ** 3).reshape(n, n, n)
+        np.testing.assert_allclose(glm, wl, rtol=1e-9, atol=1e-12)
+
+
+def test_advect_parity(ctx, orc):
+    S = ShareFields(ctx, orc, 64, 64, 21)
+    sh = S.share
+    t = S.tracer(ctx, 0.02)
+    seeds = orc.seed_generate(sh.local.offset, sh.local.size, sh.local.spacing * f32(2), 300, 21)
+    seeds["remaining_iterations"][::7] = 1
+    parts = pa.Particles.from_host(ctx, seeds)
+    state, slot, steps = t.advect(parts)
+    out = parts.download()
+    o_out, o_state, o_slot, o_steps = orc.advect(seeds, S.orc[0], neighbor_rank=sh.neighbor_ranks, dt=0.02,
+                                                 integrator=orc.RK4)
+    assert steps == o_steps
+    assert np.array_equal(state, o_state) and np.array_equal(slot, o_slot)
+    assert out.tobytes() == o_out.tobytes()
+    assert (state == 2).any() and (state == 3).any()
+
+
+def test_edge_cases(ctx, orc):
+    d, o, _ = single_block(ctx, orc, 16)
+    t = pa.particle_tracer(ctx)
+    t.set_local_vector_field(d)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.01)
+    empty = pa.Particles.from_host(ctx, np.zeros(0, pa.PARTICLE_DTYPE))
+    with pytest.raises(pa.PaError) as e:
+        t.compute_round_info(empty)
+    assert e.value.status == pa.E_INVALID
+    with pytest.raises(pa.PaError) as e:
+        t.set_integrator("runge_kutta_dormand_prince_5")
+    assert e.value.status == pa.E_UNSUPPORTED
+    # R in {0,1,2,3}, NaN and outside positions
+    p = np.zeros(6, pa.PARTICLE_DTYPE)
+    p["position"][:, :3] = 1.0
+    p["remaining_iterations"] = [0, 1, 2, 3, 9, 9]
+    p["position"][4, 0] = np.nan
+    p["position"][5, 0] = -5.0
+    p["vector_field_index"] = -1
+    chunks, res, oc, ocnt, ohand, osteps = _trace_both(ctx, orc, p, t, dict(dt=0.01, integrator=orc.RK4), o)
+    assert np.array_equal(res["counts"].download(np.int32, 6), ocnt)
+    assert list(ocnt) == [1, 1, 2, 3, 2, 2]
+    got = chunks[0][0].download(np.float32, chunks[0][1] * 4).reshape(-1, 4)
+    assert np.array_equal(got, oc[0], equal_nan=True)
+    # a particle naming a neighbour block that is not loaded is rejected
+    q = p.copy()
+    q["vector_field_index"][0] = 2
+    with pytest.raises(pa.PaError) as e:
+        t.compute_round_info(pa.Particles.from_host(ctx, q))
+    assert e.value.status == pa.E_STATE
+    # index generation of an empty array is rejected (size()-1 underflow)
+    with pytest.raises(pa.PaError):
+        pa.index_generator.generate(ctx, ctx.alloc(16), 0)
