@@ -176,7 +176,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--forms", default="fp32,bf16,fp16,fp32-consistent")
+    ap.add_argument("--forms", default="fp32,fp32-cells,bf16-cells,fp16-cells,fp32-cells-consistent")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
@@ -199,21 +199,28 @@ def main():
     assert n_seeds == 262144
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    storage_of = {"fp32": pa.FP32, "bf16": pa.BF16, "fp16": pa.FP16, "fp32-consistent": pa.FP32}
-    interp_of = {"fp32-consistent": pa.INTERP_CONSISTENT}
+    def storage_of_form(form):
+        return {"fp32": pa.FP32, "bf16": pa.BF16, "fp16": pa.FP16}[form.split("-")[0]]
+
+    def layout_of_form(form):
+        return pa.LAYOUT_CELLS if "cells" in form.split("-") else pa.LAYOUT_NODES
+
+    storage_of = {f: storage_of_form(f) for f in forms}
     fields = {}
     results = {}
     pipes = {}
     for form in forms:
         st = storage_of[form]
-        if st not in fields:
+        lay = layout_of_form(form)
+        key = (st, lay)
+        if key not in fields:
             t0 = time.perf_counter()
-            fields[st] = [pa.VectorField.synthesize(ctx, "abc", W.ABC, b.ghosted, b.int_offset, share.h, b.offset,
-                                                    b.size, b.spacing, st) for b in blocks]
-            log("fields %s: %.1f MB in %.2f s" % (form, sum(f.nbytes() for f in fields[st]) / 1e6,
+            fields[key] = [pa.VectorField.synthesize(ctx, "abc", W.ABC, b.ghosted, b.int_offset, share.h, b.offset,
+                                                     b.size, b.spacing, st).set_layout(lay) for b in blocks]
+            log("fields %s: %.1f MB in %.2f s" % (form, sum(f.nbytes() for f in fields[key]) / 1e6,
                                                   time.perf_counter() - t0))
-        interp = interp_of.get(form, pa.INTERP_REFERENCE)
-        p, res = run_form(pa, ctx, fields[st], share, st, interp, n_seeds, args.steps, args.warmup)
+        interp = pa.INTERP_CONSISTENT if "consistent" in form.split("-") else pa.INTERP_REFERENCE
+        p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup)
         log("%s: %.3f ms/step, %.3e steps/s, trace kernel %.3f ms, %s" %
             (form, res["ms_per_step"], res["value"], res["trace_ms"], res["info"]))
         results[form] = res

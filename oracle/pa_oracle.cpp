@@ -171,9 +171,10 @@ void step_frozen(int integrator, const T p[4], const T v[3], T dt, T out[4]) {
 }
 
 // Textbook RK4 (build extension, no reference counterpart): the same odeint
-// generic-RK stage sums, with the field evaluated at each stage point.  A
-// stage point outside the block ends the segment like a failed contains().
-// Returns false in that case.
+// generic-RK stage sums, with the field evaluated at each stage point.
+// Returns false when a stage point lies outside the block; the caller then
+// takes the frozen-derivative step, so the particle still reaches the face
+// and is handed off by the next contains() like in the reference.
 template <class T>
 bool step_classic(const field<T>& f, const T p[4], const T k1v[3], T dt, T out[4], bool cons) {
   const T half = T(1) / T(2);
@@ -260,10 +261,7 @@ seg_end trace_loop(const field<T>& f, const config<T>& c, const T start[4], int3
     if (stops(v, c.stop_rule, c.stop_eps)) return SEG_STOP;
     T next[4];
     if (c.integrator == ORC_CLASSIC_RK4) {
-      if (!step_classic(f, last, v, c.dt, next, c.consistent)) {
-        exit_it = int64_t(it);
-        return SEG_EXIT;
-      }
+      if (!step_classic(f, last, v, c.dt, next, c.consistent)) step_frozen(ORC_RK4, last, v, c.dt, next);
     } else {
       step_frozen(c.integrator, last, v, c.dt, next);
     }
@@ -752,20 +750,7 @@ int orc_advect(const orc_trace_config* cfg, orc_particle* particles, size_t n, i
       }
       float next[4];
       if (c.integrator == ORC_CLASSIC_RK4) {
-        if (!step_classic(f, P.position, v, c.dt, next, c.consistent)) {
-          // Stage point outside: treated like the failed contains() of the
-          // next iteration, which the classic stepper cannot reach.
-          P.remaining_iterations = int32_t(int64_t(P.remaining_iterations) - int64_t(it));
-          const int k = face_rule(f, P.position, lmask);
-          if (k < 0) {
-            P.remaining_iterations = 0;
-            state[pi] = 1;
-          } else {
-            state[pi] = 2;
-            handoff_slot[pi] = k;
-          }
-          break;
-        }
+        if (!step_classic(f, P.position, v, c.dt, next, c.consistent)) step_frozen(ORC_RK4, P.position, v, c.dt, next);
       } else {
         step_frozen(c.integrator, P.position, v, c.dt, next);
       }

@@ -50,6 +50,7 @@ INTEGRATORS = {
 }
 STOP_ISZERO, STOP_NORM = 0, 1
 INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
+LAYOUT_NODES, LAYOUT_CELLS = 0, 1
 TERMINATION_VERTEX = np.array([-2.0, -2.0, -2.0, 0.0], np.float32)
 INVALID_VERTEX = np.array([-1.0, -1.0, -1.0, 0.0], np.float32)
 INVALID_INDEX = -1
@@ -264,6 +265,12 @@ class VectorField:
         self.ctx.check(lib().pa_gpu_field_read(self.ctx.handle, self._h, out.ctypes.data_as(C.c_void_p)),
                        "field_read")
         return out
+
+    def set_layout(self, layout: int):
+        """LAYOUT_CELLS: one record of the 8 corners per cell (one cache line per gather)."""
+        self.ctx.check(lib().pa_gpu_field_set_layout(self.ctx.handle, self._h, C.c_int(layout)), "set_layout")
+        self.layout = layout
+        return self
 
     def interpolate(self, points: np.ndarray, interp: int = INTERP_REFERENCE) -> np.ndarray:
         """vector_field::contains + interpolate at points (n x 4) -> (n x 4): u, v, w, inside."""

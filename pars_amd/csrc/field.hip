@@ -89,6 +89,51 @@ __global__ void k_synth(void* __restrict__ nodes, int64_t n, int d1, int d2, int
   }
 }
 
+// Cell records: corners c000, c001, c010, c011, c100, c101, c110, c111 (the
+// order vector_field::interpolate lerps them), xyz each.
+__global__ void k_cells(const void* __restrict__ nodes, void* __restrict__ cells, int64_t ncells, int d1, int d2,
+                        int storage) {
+  const int64_t cs1 = d2 - 1, cs0 = int64_t(d1 - 1) * (d2 - 1);
+  const int64_t ns1 = d2, ns0 = int64_t(d1) * d2;
+  for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncells; c += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = c / cs0, j = (c / cs1) % (d1 - 1), k = c % cs1;
+    const int64_t n0 = i * ns0 + j * ns1 + k;
+    const int64_t corner[8] = {n0, n0 + 1, n0 + ns1, n0 + ns1 + 1, n0 + ns0, n0 + ns0 + 1, n0 + ns0 + ns1,
+                               n0 + ns0 + ns1 + 1};
+    if (storage == PA_STORAGE_FP32) {
+      float r[32];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float4 v = static_cast<const float4*>(nodes)[corner[q]];
+        r[3 * q] = v.x;
+        r[3 * q + 1] = v.y;
+        r[3 * q + 2] = v.z;
+      }
+#pragma unroll
+      for (int q = 24; q < 32; ++q) r[q] = 0.0f;
+      float4* out = static_cast<float4*>(cells) + c * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) out[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    } else {
+      uint32_t hw[32];  // 16-bit halves
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint2 u = static_cast<const uint2*>(nodes)[corner[q]];
+        hw[3 * q] = u.x & 0xffffu;
+        hw[3 * q + 1] = u.x >> 16;
+        hw[3 * q + 2] = u.y & 0xffffu;
+      }
+#pragma unroll
+      for (int q = 24; q < 32; ++q) hw[q] = 0u;
+      uint4* out = static_cast<uint4*>(cells) + c * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        out[q] = make_uint4(hw[8 * q] | (hw[8 * q + 1] << 16), hw[8 * q + 2] | (hw[8 * q + 3] << 16),
+                            hw[8 * q + 4] | (hw[8 * q + 5] << 16), hw[8 * q + 6] | (hw[8 * q + 7] << 16));
+    }
+  }
+}
+
 int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
@@ -127,6 +172,9 @@ pa_status make_field(pa_gpu_ctx* ctx, const int32_t dims[3], const float offset[
   }
   d.stride1 = dims[2];
   d.stride0 = int64_t(dims[1]) * dims[2];
+  d.cells = nullptr;
+  d.cstride0 = d.cstride1 = 0;
+  d.layout = PA_LAYOUT_NODES;
   *out = f;
   return PA_OK;
 }
@@ -209,9 +257,35 @@ pa_status pa_gpu_field_read(pa_gpu_ctx* ctx, const pa_gpu_field* field, float* h
   return PA_OK;
 }
 
+pa_status pa_gpu_field_set_layout(pa_gpu_ctx* ctx, pa_gpu_field* field, pa_layout layout) {
+  if (!ctx || !field) return PA_E_INVALID;
+  FieldDesc& d = field->desc;
+  if (layout == PA_LAYOUT_NODES) {
+    d.layout = PA_LAYOUT_NODES;
+    return PA_OK;
+  }
+  if (layout != PA_LAYOUT_CELLS) return fail(ctx, PA_E_INVALID, "unknown layout");
+  if (!field->d_cells) {
+    const int64_t ncells = int64_t(d.dims[0] - 1) * (d.dims[1] - 1) * (d.dims[2] - 1);
+    field->cell_bytes = size_t(ncells) * (d.storage == PA_STORAGE_FP32 ? 128 : 64);
+    PA_HIP(ctx, hipMalloc(&field->d_cells, field->cell_bytes));
+    launch_begin(ctx, "field_cells");
+    hipLaunchKernelGGL(k_cells, dim3(grid_for(ncells)), dim3(256), 0, ctx->stream, field->d_nodes, field->d_cells,
+                       ncells, d.dims[1], d.dims[2], d.storage);
+    launch_end(ctx);
+    PA_HIP(ctx, hipGetLastError());
+    PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  d.cells = field->d_cells;
+  d.cstride1 = d.dims[2] - 1;
+  d.cstride0 = int64_t(d.dims[1] - 1) * (d.dims[2] - 1);
+  d.layout = PA_LAYOUT_CELLS;
+  return PA_OK;
+}
+
 pa_status pa_gpu_field_bytes(const pa_gpu_field* field, size_t* bytes) {
   if (!field || !bytes) return PA_E_INVALID;
-  *bytes = field->bytes;
+  *bytes = field->bytes + field->cell_bytes;
   return PA_OK;
 }
 
@@ -219,6 +293,7 @@ pa_status pa_gpu_field_destroy(pa_gpu_ctx* ctx, pa_gpu_field* field) {
   if (!ctx || !field) return PA_E_INVALID;
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   PA_HIP(ctx, hipFree(field->d_nodes));
+  if (field->d_cells) PA_HIP(ctx, hipFree(field->d_cells));
   delete field;
   return PA_OK;
 }

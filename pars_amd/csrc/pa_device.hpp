@@ -27,6 +27,15 @@ __device__ __forceinline__ float f16_hi(uint32_t u) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16)));
 }
 
+// Template storage code S = storage + 3 * layout.
+constexpr int storage_of(int S) { return S % 3; }
+constexpr int layout_of(int S) { return S / 3; }
+
+__device__ __forceinline__ float dec16(uint32_t h, int storage) {
+  if (storage == PA_STORAGE_BF16) return __uint_as_float(h << 16);
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h & 0xffffu)));
+}
+
 // One node of a block, decoded to FP32 (x, y, z); the 4th component is
 // padding.
 template <int S>
@@ -104,20 +113,46 @@ __device__ __forceinline__ void interpolate(const FieldDesc& f, const float d[3]
     }
     u[i] = 1.0f - w[i];
   }
-  const int64_t i000 = static_cast<int64_t>(m[0]) * f.stride0 + static_cast<int64_t>(m[1]) * f.stride1 + m[2];
-  const int64_t i010 = i000 + f.stride1;
-  const int64_t i100 = i000 + f.stride0;
-  const int64_t i110 = i100 + f.stride1;
   float a0, a1, a2, b0, b1, b2, c0, c1, c2, e0, e1, e2;
   float g0, g1, g2, h0, h1, h2, k0, k1, k2, l0, l1, l2;
-  load_node<S>(f.data, i000, a0, a1, a2);
-  load_node<S>(f.data, i000 + 1, b0, b1, b2);
-  load_node<S>(f.data, i010, c0, c1, c2);
-  load_node<S>(f.data, i010 + 1, e0, e1, e2);
-  load_node<S>(f.data, i100, g0, g1, g2);
-  load_node<S>(f.data, i100 + 1, h0, h1, h2);
-  load_node<S>(f.data, i110, k0, k1, k2);
-  load_node<S>(f.data, i110 + 1, l0, l1, l2);
+  if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
+    // One record holds c000 c001 c010 c011 c100 c101 c110 c111 (xyz each).
+    const int64_t ci = static_cast<int64_t>(m[0]) * f.cstride0 + static_cast<int64_t>(m[1]) * f.cstride1 + m[2];
+    if constexpr (storage_of(S) == PA_STORAGE_FP32) {
+      const float4* r = static_cast<const float4*>(f.cells) + ci * 8;
+      const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+      a0 = r0.x; a1 = r0.y; a2 = r0.z; b0 = r0.w; b1 = r1.x; b2 = r1.y;
+      c0 = r1.z; c1 = r1.w; c2 = r2.x; e0 = r2.y; e1 = r2.z; e2 = r2.w;
+      g0 = r3.x; g1 = r3.y; g2 = r3.z; h0 = r3.w; h1 = r4.x; h2 = r4.y;
+      k0 = r4.z; k1 = r4.w; k2 = r5.x; l0 = r5.y; l1 = r5.z; l2 = r5.w;
+    } else {
+      constexpr int st = storage_of(S);
+      const uint4* r = static_cast<const uint4*>(f.cells) + ci * 4;
+      const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
+      a0 = dec16(r0.x, st); a1 = dec16(r0.x >> 16, st); a2 = dec16(r0.y, st);
+      b0 = dec16(r0.y >> 16, st); b1 = dec16(r0.z, st); b2 = dec16(r0.z >> 16, st);
+      c0 = dec16(r0.w, st); c1 = dec16(r0.w >> 16, st); c2 = dec16(r1.x, st);
+      e0 = dec16(r1.x >> 16, st); e1 = dec16(r1.y, st); e2 = dec16(r1.y >> 16, st);
+      g0 = dec16(r1.z, st); g1 = dec16(r1.z >> 16, st); g2 = dec16(r1.w, st);
+      h0 = dec16(r1.w >> 16, st); h1 = dec16(r2.x, st); h2 = dec16(r2.x >> 16, st);
+      k0 = dec16(r2.y, st); k1 = dec16(r2.y >> 16, st); k2 = dec16(r2.z, st);
+      l0 = dec16(r2.z >> 16, st); l1 = dec16(r2.w, st); l2 = dec16(r2.w >> 16, st);
+    }
+  } else {
+    constexpr int SN = storage_of(S);
+    const int64_t i000 = static_cast<int64_t>(m[0]) * f.stride0 + static_cast<int64_t>(m[1]) * f.stride1 + m[2];
+    const int64_t i010 = i000 + f.stride1;
+    const int64_t i100 = i000 + f.stride0;
+    const int64_t i110 = i100 + f.stride1;
+    load_node<SN>(f.data, i000, a0, a1, a2);
+    load_node<SN>(f.data, i000 + 1, b0, b1, b2);
+    load_node<SN>(f.data, i010, c0, c1, c2);
+    load_node<SN>(f.data, i010 + 1, e0, e1, e2);
+    load_node<SN>(f.data, i100, g0, g1, g2);
+    load_node<SN>(f.data, i100 + 1, h0, h1, h2);
+    load_node<SN>(f.data, i110, k0, k1, k2);
+    load_node<SN>(f.data, i110 + 1, l0, l1, l2);
+  }
   // c00 = lerp(c000, c001, w2), c01 = lerp(c010, c011, w2), ...
   const float c00x = lerp1(a0, b0, w[2], u[2]), c00y = lerp1(a1, b1, w[2], u[2]), c00z = lerp1(a2, b2, w[2], u[2]);
   const float c01x = lerp1(c0, e0, w[2], u[2]), c01y = lerp1(c1, e1, w[2], u[2]), c01z = lerp1(c2, e2, w[2], u[2]);
@@ -206,8 +241,7 @@ __device__ __forceinline__ int face_rule(const FieldDesc& f, float4 p, int mask)
 __device__ __forceinline__ float4 termination_vertex() { return make_float4(-2.0f, -2.0f, -2.0f, 0.0f); }
 
 // One field evaluation plus one step from `last`.  Returns 0 = stepped
-// (next valid), 1 = left the block (contains failed, or a classic stage
-// point did), 2 = stop rule.
+// (next valid), 1 = left the block (contains failed), 2 = stop rule.
 template <int S, int MODE, int INTERP, int STOP>
 __device__ __forceinline__ int advance(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next) {
   float d[3], q[3], v[3];
@@ -215,7 +249,9 @@ __device__ __forceinline__ int advance(const TraceConfig& c, const FieldDesc& f,
   interpolate<S, INTERP>(f, d, q, v);
   if (stops<STOP>(v, c.stop_eps2)) return 2;
   if constexpr (MODE == STEP_CLASSIC) {
-    if (!step_classic<S, INTERP>(c, f, last, v, next)) return 1;
+    // A stage point outside the block: take the frozen-derivative step so
+    // the particle reaches the face and is handed off by the next contains.
+    if (!step_classic<S, INTERP>(c, f, last, v, next)) next = step_frozen<STEP_RK4>(c, last, v);
   } else {
     next = step_frozen<MODE>(c, last, v);
   }
@@ -228,7 +264,7 @@ using IC = std::integral_constant<int, V>;
 // Calls f(IC<storage>, IC<mode>, IC<interp>) with the run-time choices as
 // compile-time constants (one kernel instantiation per combination).
 template <class F>
-inline void dispatch(int storage, int mode, int interp, F&& f) {
+inline void dispatch(int storage, int mode, int interp, F&& f, int layout = PA_LAYOUT_NODES) {
   auto with_i = [&](auto s, auto m) {
     if (interp == PA_INTERP_CONSISTENT) f(s, m, IC<PA_INTERP_CONSISTENT>{});
     else f(s, m, IC<PA_INTERP_REFERENCE>{});
@@ -238,9 +274,15 @@ inline void dispatch(int storage, int mode, int interp, F&& f) {
     else if (mode == STEP_CLASSIC) with_i(s, IC<STEP_CLASSIC>{});
     else with_i(s, IC<STEP_RK4>{});
   };
-  if (storage == PA_STORAGE_BF16) with_m(IC<PA_STORAGE_BF16>{});
-  else if (storage == PA_STORAGE_FP16) with_m(IC<PA_STORAGE_FP16>{});
-  else with_m(IC<PA_STORAGE_FP32>{});
+  if (layout == PA_LAYOUT_CELLS) {
+    if (storage == PA_STORAGE_BF16) with_m(IC<PA_STORAGE_BF16 + 3>{});
+    else if (storage == PA_STORAGE_FP16) with_m(IC<PA_STORAGE_FP16 + 3>{});
+    else with_m(IC<PA_STORAGE_FP32 + 3>{});
+  } else {
+    if (storage == PA_STORAGE_BF16) with_m(IC<PA_STORAGE_BF16>{});
+    else if (storage == PA_STORAGE_FP16) with_m(IC<PA_STORAGE_FP16>{});
+    else with_m(IC<PA_STORAGE_FP32>{});
+  }
 }
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {

@@ -30,6 +30,10 @@ struct FieldDesc {
   float size[3];
   int64_t stride0;    // dims[1] * dims[2]
   int64_t stride1;    // dims[2]
+  const void* cells;  // cell records (PA_LAYOUT_CELLS) or null
+  int64_t cstride0;   // (dims[1]-1) * (dims[2]-1)
+  int64_t cstride1;   // dims[2]-1
+  int32_t layout;     // pa_layout
 };
 
 // Per-launch tracer configuration.  Slot 0 is the own block (vector field
@@ -54,6 +58,8 @@ struct pa_gpu_field {
   pa_amd::FieldDesc desc;
   void* d_nodes = nullptr;
   size_t bytes = 0;
+  void* d_cells = nullptr;
+  size_t cell_bytes = 0;
 };
 
 struct pa_gpu_ctx {

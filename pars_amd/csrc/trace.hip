@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TraceConfig cfg, const p
     float4 last = make_float4(p.position[0], p.position[1], p.position[2], p.position[3]);
     out[0] = last;
     const int b = p.vector_field_index;
-    const FieldDesc& f = cfg.field[b + 1];
+    const FieldDesc f = cfg.field[b + 1];  // register copy: no descriptor loads in the step loop
     const int R = p.remaining_iterations;
     int it = 1;
     bool exited = false;
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, c
     int64_t base = 0;
     int64_t used = 1;
     for (;;) {
-      const FieldDesc& f = cfg.field[b + 1];
+      const FieldDesc f = cfg.field[b + 1];  // register copy: no descriptor loads in the step loop
       int it = 1;
       bool exited = false;
       for (; it < R; ++it) {
@@ -390,9 +390,10 @@ pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg) {
   int storage = -1;
   auto put = [&](int slot, const pa_gpu_field* f) -> pa_status {
     if (!f) return PA_OK;
-    if (storage >= 0 && f->desc.storage != storage)
-      return fail(ctx, PA_E_UNSUPPORTED, "all blocks of one tracer must share one storage type");
-    storage = f->desc.storage;
+    const int code = f->desc.storage + 3 * f->desc.layout;
+    if (storage >= 0 && code != storage)
+      return fail(ctx, PA_E_UNSUPPORTED, "all blocks of one tracer must share one storage type and layout");
+    storage = code;
     cfg->field[slot] = f->desc;
     cfg->has_field[slot] = 1;
     return PA_OK;
@@ -539,11 +540,12 @@ pa_status pa_gpu_field_interpolate(pa_gpu_ctx* ctx, const pa_gpu_field* field, p
   if (interp != PA_INTERP_REFERENCE && interp != PA_INTERP_CONSISTENT)
     return fail(ctx, PA_E_INVALID, "unknown interpolation mode");
   launch_begin(ctx, "interpolate");
-  dispatch(field->desc.storage, STEP_RK4, interp, [&](auto S, auto, auto I) {
-    hipLaunchKernelGGL((k_interpolate<decltype(S)::value, decltype(I)::value>), dim3(grid_of(int64_t(n))),
-                       dim3(kBlock), 0, ctx->stream, field->desc, reinterpret_cast<const float4*>(d_points),
-                       int64_t(n), reinterpret_cast<float4*>(d_out));
-  });
+  dispatch(field->desc.storage, STEP_RK4, interp, [&](auto S, auto M, auto I) {
+    if constexpr (decltype(M)::value == STEP_RK4)
+      hipLaunchKernelGGL((k_interpolate<decltype(S)::value, decltype(I)::value>), dim3(grid_of(int64_t(n))),
+                         dim3(kBlock), 0, ctx->stream, field->desc, reinterpret_cast<const float4*>(d_points),
+                         int64_t(n), reinterpret_cast<float4*>(d_out));
+  }, field->desc.layout);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;
@@ -640,7 +642,7 @@ pa_status pa_gpu_tracer_trace(pa_gpu_tracer* t, const pa_particle* d_particles, 
                        dim3(kBlock), 0, ctx->stream, cfg, d_particles, int64_t(n),
                        reinterpret_cast<float4* const*>(w), int64_t(ppic), int64_t(maxR), d_vertex_counts, tmp, code,
                        steps, t->stop_rule);
-  });
+  }, cfg.field[0].layout);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   if (handoff) PA_TRY(group_handoffs(ctx, code, tmp, int64_t(n), handoff, w + table + tmp_bytes));
@@ -682,7 +684,7 @@ pa_status pa_gpu_tracer_trace_union(pa_gpu_tracer* t, const pa_particle* d_parti
                        dim3(grid_of(int64_t(n))), dim3(kBlock), 0, ctx->stream, cfg, d_particles, int64_t(n),
                        reinterpret_cast<float4*>(d_vertices), int64_t(region_slots), d_vertex_counts, d_continuation,
                        ncont, steps, t->stop_rule, reinterpret_cast<float4*>(d_endpoints), d_seed_steps);
-  });
+  }, cfg.field[0].layout);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   unsigned long long h[2] = {0, 0};
@@ -714,7 +716,7 @@ pa_status pa_gpu_advect(pa_gpu_tracer* t, pa_particle* d_particles, size_t n, in
     hipLaunchKernelGGL((k_advect<decltype(S)::value, decltype(M)::value, decltype(I)::value>), dim3(grid_of(int64_t(n))),
                        dim3(kBlock), 0, ctx->stream, cfg, d_particles, int64_t(n), d_state, d_handoff_slot, steps,
                        t->stop_rule);
-  });
+  }, cfg.field[0].layout);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   unsigned long long s = 0;
@@ -765,7 +767,7 @@ pa_status pa_gpu_flow_map_generate(pa_gpu_tracer* t, const int32_t block_size[3]
                          dim3(grid_of(int64_t(ns))), dim3(kBlock), 0, ctx->stream, cfg, int64_t(ns), sd[1], sd[2],
                          f.offset[0], f.offset[1], f.offset[2], spacing[0], spacing[1], spacing[2],
                          int32_t(iterations), d_flow_map, dims_out[0], dims_out[1], dims_out[2], steps, t->stop_rule);
-    });
+    }, cfg.field[0].layout);
     launch_end(ctx);
   }
   PA_HIP(ctx, hipGetLastError());

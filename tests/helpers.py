@@ -11,7 +11,8 @@ f32 = np.float32
 class ShareFields:
     """Own block + 6 neighbours of one rank, on the device and in the oracle."""
 
-    def __init__(self, ctx, orc, domain_n, ranks, rank, storage=pa.FP32, field="abc", modes=None):
+    def __init__(self, ctx, orc, domain_n, ranks, rank, storage=pa.FP32, field="abc", modes=None,
+                 layout=pa.LAYOUT_NODES):
         self.share = W.share(domain_n, ranks, rank)
         sh = self.share
         infos = [sh.local] + list(sh.neighbors)
@@ -26,6 +27,7 @@ class ShareFields:
                 continue
             u64 = W.abc_block_f64(info, sh.h) if field == "abc" else W.fourier_block_f64(info, sh.h, modes)
             d = pa.VectorField.from_host(ctx, u64.astype(np.float32), info.offset, info.size, info.spacing, storage)
+            d.set_layout(layout)
             self.dev.append(d)
             self.orc.append(orc.Field(d.read(), info.offset, info.size, info.spacing))
             self.f64.append(orc.Field64(u64, info.offset.astype(np.float64), info.size.astype(np.float64),
@@ -54,7 +56,7 @@ class ShareFields:
                     interp=interp)
 
 
-def single_block(ctx, orc, n, storage=pa.FP32, h=None):
+def single_block(ctx, orc, n, storage=pa.FP32, h=None, layout=pa.LAYOUT_NODES):
     """ABC on n^3 nodes over [0, 2pi], one block without neighbours."""
     h = h if h is not None else 2 * np.pi / (n - 1)
     info = W.BlockInfo(0, (0, 0, 0), (0, 0, 0), (n, n, n))
@@ -64,6 +66,7 @@ def single_block(ctx, orc, n, storage=pa.FP32, h=None):
     size = np.array([f32(n) * hf] * 3, np.float32)  # data_io: size = block * spacing
     sp = np.array([hf] * 3, np.float32)
     d = pa.VectorField.from_host(ctx, u64.astype(np.float32), off, size, sp, storage)
+    d.set_layout(layout)
     o = orc.Field(d.read(), off, size, sp)
     o64 = orc.Field64(u64, off.astype(np.float64), size.astype(np.float64), [h] * 3)
     return d, o, o64

@@ -51,8 +51,9 @@ def test_field_storage_roundtrip(ctx, storage):
 
 @pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("interp", [pa.INTERP_REFERENCE, pa.INTERP_CONSISTENT])
-def test_interpolate_bitexact(ctx, orc, storage, interp):
-    d, o, _ = single_block(ctx, orc, 33, storage)
+@pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
+def test_interpolate_bitexact(ctx, orc, storage, interp, layout):
+    d, o, _ = single_block(ctx, orc, 33, storage, layout=layout)
     rng = np.random.default_rng(12)
     h = o.spacing[0]
     n = 20000
@@ -106,11 +107,12 @@ def test_trace_config1_bitexact(ctx, orc):
 @pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("integ", [pa.RUNGE_KUTTA_4, pa.EULER, pa.CLASSIC_RK4])
 @pytest.mark.parametrize("interp", [pa.INTERP_REFERENCE, pa.INTERP_CONSISTENT])
-def test_trace_share_bitexact(ctx, orc, storage, integ, interp):
+@pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
+def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout):
     """One rank's share (own block + 6 neighbours, 4x4x4 grid of a 64^3 ABC
     domain, rank 21), own-block seeds plus load-balanced particles that live in
     neighbour blocks; slots, counts and both hand-off maps compared."""
-    S = ShareFields(ctx, orc, 64, 64, 21, storage)
+    S = ShareFields(ctx, orc, 64, 64, 21, storage, layout=layout)
     sh = S.share
     dt = 0.02
     t = S.tracer(ctx, dt, integ, interp)
@@ -138,10 +140,11 @@ def test_trace_share_bitexact(ctx, orc, storage, integ, interp):
 
 @pytest.mark.parametrize("storage", [pa.FP32, pa.BF16])
 @pytest.mark.parametrize("region_extra", [64, 1])
-def test_trace_union_bitexact(ctx, orc, storage, region_extra):
+@pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
+def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout):
     """Union-of-7 continuation vs the oracle's round emulation: every seed's
     segments, in order; a tiny region forces the continuation path."""
-    S = ShareFields(ctx, orc, 64, 64, 21, storage)
+    S = ShareFields(ctx, orc, 64, 64, 21, storage, layout=layout)
     sh = S.share
     dt, R = 0.02, 400
     t = S.tracer(ctx, dt)
@@ -211,9 +214,10 @@ def test_prune_index_color_parity(ctx, orc):
 
 
 @pytest.mark.parametrize("storage", STORAGES)
-def test_flow_map_and_ftle(ctx, orc, storage):
+@pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
+def test_flow_map_and_ftle(ctx, orc, storage, layout):
     n, R, dt = 40, 101, 0.01
-    d, o, _ = single_block(ctx, orc, n, storage)
+    d, o, _ = single_block(ctx, orc, n, storage, layout=layout)
     fmg = pa.flow_map_generator(ctx)
     fmg.set_vector_field(d)
     fmg.set_integrator("runge_kutta_4")
