@@ -47,8 +47,9 @@ def log(*a):
 class Pipeline:
     """Device buffers and calls of one step, reused across steps."""
 
-    def __init__(self, pa, ctx, fields, share, storage, interp, n_max):
+    def __init__(self, pa, ctx, fields, share, storage, interp, n_max, order="step"):
         self.pa, self.ctx, self.lib = pa, ctx, pa.lib()
+        self.order = order
         self.share = share
         t = pa.particle_tracer(ctx)
         t.set_local_vector_field(fields[0])
@@ -78,13 +79,19 @@ class Pipeline:
                                            C.c_void_p(self.seeds.ptr), C.c_size_t(self.seeds.nbytes // 48),
                                            C.byref(n)), "seeds")
         parts = pa.Particles(self.seeds, n.value)
-        r = self.tracer.trace_union(parts, self.region, buffers=self.ub)
+        r = self.tracer.trace_union(parts, self.region, buffers=self.ub, order=self.order)
         if r["continuation"].n:
             raise RuntimeError("segment region overflow: %d seeds need a continuation pass" % r["continuation"].n)
         m = C.c_size_t()
-        ctx.check(lib.pa_gpu_prune_counted(ctx.handle, C.c_void_p(self.ub["vertices"].ptr), C.c_size_t(self.region),
-                                           C.c_void_p(self.ub["counts"].ptr), n, C.c_void_p(self.pruned.ptr),
-                                           C.c_size_t(self.cap), C.byref(m), C.c_void_p(self.offs.ptr)), "prune")
+        if self.order == "step":
+            ctx.check(lib.pa_gpu_prune_counted_step_major(
+                ctx.handle, C.c_void_p(self.ub["vertices"].ptr), n, C.c_void_p(self.ub["counts"].ptr),
+                C.c_void_p(self.pruned.ptr), C.c_size_t(self.cap), C.byref(m), C.c_void_p(self.offs.ptr)), "prune")
+        else:
+            ctx.check(lib.pa_gpu_prune_counted(
+                ctx.handle, C.c_void_p(self.ub["vertices"].ptr), C.c_size_t(self.region),
+                C.c_void_p(self.ub["counts"].ptr), n, C.c_void_p(self.pruned.ptr), C.c_size_t(self.cap),
+                C.byref(m), C.c_void_p(self.offs.ptr)), "prune")
         ni = C.c_size_t()
         ctx.check(lib.pa_gpu_index_generate(ctx.handle, C.c_void_p(self.pruned.ptr), m, C.c_void_p(self.indices.ptr),
                                             C.byref(ni)), "index")
@@ -93,8 +100,8 @@ class Pipeline:
         return dict(particles=n.value, steps=r["total_steps"], vertices=m.value, indices=ni.value)
 
 
-def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup):
-    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds)
+def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order):
+    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order)
     for _ in range(warmup):
         info = p.step()
     ctx.synchronize()
@@ -220,7 +227,8 @@ def main():
             log("fields %s: %.1f MB in %.2f s" % (form, sum(f.nbytes() for f in fields[key]) / 1e6,
                                                   time.perf_counter() - t0))
         interp = pa.INTERP_CONSISTENT if "consistent" in form.split("-") else pa.INTERP_REFERENCE
-        p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup)
+        order = "seed" if "seedmajor" in form.split("-") else "step"
+        p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order)
         log("%s: %.3f ms/step, %.3e steps/s, trace kernel %.3f ms, %s" %
             (form, res["ms_per_step"], res["value"], res["trace_ms"], res["info"]))
         results[form] = res

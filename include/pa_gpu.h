@@ -233,6 +233,21 @@ pa_status   pa_gpu_tracer_set_step_size(pa_gpu_tracer* tracer, float step_size);
 pa_status   pa_gpu_tracer_set_stop_rule(pa_gpu_tracer* tracer, pa_stop_rule rule, float eps);
 /* Extension: interpolation weights (default PA_INTERP_REFERENCE). */
 pa_status   pa_gpu_tracer_set_interpolation(pa_gpu_tracer* tracer, pa_interp interp);
+/* Extension: tracer engine.  PA_ENGINE_LANE runs one lane per particle
+ * with per-lane gathers; PA_ENGINE_COOPERATIVE keeps one lane per
+ * particle for the arithmetic but gathers the 8 corners cooperatively (groups
+ * of lanes load one particle's cell record or z-pairs as coalesced 16-B
+ * pieces, redistributed through LDS) and refills finished lanes from a work
+ * counter.  Both produce identical results; PA_ENGINE_LANE is the default
+ * (faster on gfx950).  waves = resident waves of the cooperative engine (0 =
+ * automatic). */
+typedef enum pa_engine { PA_ENGINE_LANE = 0, PA_ENGINE_COOPERATIVE = 1 } pa_engine;
+pa_status   pa_gpu_tracer_set_engine(pa_gpu_tracer* tracer, pa_engine engine, int32_t waves);
+/* Extension: XCD-aware workgroup order of the lane engine (default on):
+ * the 8 XCDs, which take workgroups round-robin, each process a contiguous
+ * eighth of the particle list, so particles seeded close together share an
+ * L2.  Results are unchanged. */
+pa_status   pa_gpu_tracer_set_xcd_remap(pa_gpu_tracer* tracer, int32_t enable);
 /* Union-of-7 topology: bit d of face_mask[k] set when neighbour block k has
  * its own neighbour across face d (partitioner.cpp:47-52 order). */
 pa_status   pa_gpu_tracer_set_neighbor_face_masks(pa_gpu_tracer* tracer, const int32_t face_mask[6]);
@@ -276,6 +291,25 @@ pa_status   pa_gpu_tracer_trace_union(pa_gpu_tracer* tracer, const pa_particle* 
                                       size_t region_slots, float* d_vertices, int32_t* d_vertex_counts,
                                       pa_particle* d_continuation, size_t* n_continuation, float* d_endpoints,
                                       int32_t* d_seed_steps, uint64_t* total_steps);
+
+/* Extension: order of the per-seed slots of the union trace.
+ * PA_SLOTS_SEED_MAJOR: slot s of seed p at d_vertices[p * region_slots + s]
+ * (the layout above).  PA_SLOTS_STEP_MAJOR: at d_vertices[s * n + p]; the
+ * lanes of a wave step in lockstep, so every vertex store is one contiguous
+ * run, which is what the memory system rewards.  Pack either into the
+ * reference's pruned layout with pa_gpu_prune_counted /
+ * pa_gpu_prune_counted_step_major.  Step-major needs PA_ENGINE_LANE. */
+typedef enum pa_slot_order { PA_SLOTS_SEED_MAJOR = 0, PA_SLOTS_STEP_MAJOR = 1 } pa_slot_order;
+pa_status   pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
+                                             size_t region_slots, pa_slot_order order, float* d_vertices,
+                                             int32_t* d_vertex_counts, pa_particle* d_continuation,
+                                             size_t* n_continuation, float* d_endpoints, int32_t* d_seed_steps,
+                                             uint64_t* total_steps);
+/* Counted prune of step-major slots (row stride n seeds) into the pruned
+ * particle-major layout; same outputs as pa_gpu_prune_counted. */
+pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots, size_t n,
+                                            const int32_t* d_vertex_counts, float* d_out, size_t capacity,
+                                            size_t* n_out, int64_t* d_offsets);
 
 /* prune (particle_tracer.cpp:402-409): order-preserving removal of
  * invalid_vertex from one chunk.  *n_out receives the kept count. */

@@ -51,6 +51,9 @@ INTEGRATORS = {
 STOP_ISZERO, STOP_NORM = 0, 1
 INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
 LAYOUT_NODES, LAYOUT_CELLS = 0, 1
+ENGINE_LANE, ENGINE_COOPERATIVE = 0, 1
+SLOTS_SEED_MAJOR, SLOTS_STEP_MAJOR = 0, 1
+SLOT_ORDERS = {"seed": SLOTS_SEED_MAJOR, "step": SLOTS_STEP_MAJOR}
 TERMINATION_VERTEX = np.array([-2.0, -2.0, -2.0, 0.0], np.float32)
 INVALID_VERTEX = np.array([-1.0, -1.0, -1.0, 0.0], np.float32)
 INVALID_INDEX = -1
@@ -382,6 +385,13 @@ class particle_tracer:
     def set_step_size(self, step_size: float):
         self.ctx.check(lib().pa_gpu_tracer_set_step_size(self._h, C.c_float(step_size)))
 
+    def set_engine(self, engine: int, waves: int = 0):
+        """ENGINE_COOPERATIVE (default) or ENGINE_LANE; waves = resident waves (0 = automatic)."""
+        self.ctx.check(lib().pa_gpu_tracer_set_engine(self._h, C.c_int(engine), C.c_int32(waves)), "set_engine")
+
+    def set_xcd_remap(self, enable: bool):
+        self.ctx.check(lib().pa_gpu_tracer_set_xcd_remap(self._h, C.c_int32(int(enable))), "set_xcd_remap")
+
     def set_interpolation(self, interp: int):
         self.ctx.check(lib().pa_gpu_tracer_set_interpolation(self._h, C.c_int(interp)), "set_interpolation")
 
@@ -430,14 +440,16 @@ class particle_tracer:
             out["handoff_sources"] = src.download(np.int64, tot) if tot else np.zeros(0, np.int64)
         return out
 
-    def trace_union(self, particles: Particles, region_slots: int, buffers=None, endpoints=False):
-        """Union-of-7 trace; buffers (from union_buffers) may be reused across calls."""
+    def trace_union(self, particles: Particles, region_slots: int, buffers=None, endpoints=False, order="seed"):
+        """Union-of-7 trace; buffers (from union_buffers) may be reused across calls.
+
+        order "seed": slot s of seed p at p * region_slots + s; "step": at s * n + p."""
         b = buffers or self.union_buffers(particles.n, region_slots, endpoints)
         ncont = C.c_size_t()
         steps = C.c_uint64()
-        self.ctx.check(lib().pa_gpu_tracer_trace_union(
+        self.ctx.check(lib().pa_gpu_tracer_trace_union_layout(
             self._h, C.c_void_p(particles.ptr), C.c_size_t(particles.n), C.c_size_t(region_slots),
-            C.c_void_p(b["vertices"].ptr), C.c_void_p(b["counts"].ptr), C.c_void_p(b["cont"].ptr), C.byref(ncont),
+            C.c_int(SLOT_ORDERS[order]), C.c_void_p(b["vertices"].ptr), C.c_void_p(b["counts"].ptr), C.c_void_p(b["cont"].ptr), C.byref(ncont),
             C.c_void_p(b["endpoints"].ptr) if b.get("endpoints") else None,
             C.c_void_p(b["seed_steps"].ptr) if b.get("seed_steps") else None, C.byref(steps)), "trace_union")
         return dict(vertices=b["vertices"], counts=b["counts"], continuation=Particles(b["cont"], ncont.value),
@@ -480,6 +492,18 @@ class particle_tracer:
         ctx.check(lib().pa_gpu_prune_counted(ctx.handle, C.c_void_p(slots), C.c_size_t(stride_slots),
                                              C.c_void_p(counts.ptr), C.c_size_t(n), C.c_void_p(out.ptr),
                                              C.c_size_t(capacity), C.byref(m), C.c_void_p(offs.ptr)), "prune_counted")
+        return out, m.value, offs
+
+    @staticmethod
+    def prune_counted_step_major(ctx: Context, slots: int, counts: DeviceBuffer, n: int, capacity: int,
+                                 out: DeviceBuffer = None, offs: DeviceBuffer = None):
+        """Pack step-major union slots (row stride n) into the pruned particle-major layout."""
+        out = ctx.alloc(max(capacity, 1) * 16) if out is None else out
+        offs = ctx.alloc((n + 1) * 8) if offs is None else offs
+        m = C.c_size_t()
+        ctx.check(lib().pa_gpu_prune_counted_step_major(
+            ctx.handle, C.c_void_p(slots), C.c_size_t(n), C.c_void_p(counts.ptr), C.c_void_p(out.ptr),
+            C.c_size_t(capacity), C.byref(m), C.c_void_p(offs.ptr)), "prune_counted_step_major")
         return out, m.value, offs
 
 

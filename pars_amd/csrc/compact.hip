@@ -218,6 +218,49 @@ __global__ __launch_bounds__(kBlock) void k_copy_counted(const float4* __restric
   }
 }
 
+// Step-major slots (slot s of seed p at s * n + p) into the pruned layout:
+// a workgroup takes 64 consecutive seeds and moves 64-slot tiles through LDS,
+// reading whole rows (1 KiB per wave) and writing each seed's run contiguously.
+constexpr int kTile = 64;
+
+__global__ __launch_bounds__(kBlock) void k_copy_counted_step_major(const float4* __restrict__ slots, int64_t n,
+                                                                    const int32_t* __restrict__ counts,
+                                                                    const int64_t* __restrict__ offs,
+                                                                    float4* __restrict__ out) {
+  __shared__ float4 tile[kTile][kTile + 1];
+  __shared__ int32_t cnt[kTile];
+  __shared__ int64_t dst[kTile];
+  __shared__ int32_t wave_max[kBlock / 64];
+  const int64_t p0 = int64_t(blockIdx.x) * kTile;
+  const int t = threadIdx.x, lane = lane_id(), w = wave_id();
+  if (t < kTile) {
+    const int64_t p = p0 + t;
+    cnt[t] = p < n ? counts[p] : 0;
+    dst[t] = p < n ? offs[p] : 0;
+  }
+  __syncthreads();
+  int32_t m = cnt[lane];
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+  if (lane == 0) wave_max[w] = m;
+  __syncthreads();
+  const int32_t cmax = wave_max[0];
+  for (int32_t s0 = 0; s0 < cmax; s0 += kTile) {
+    // read: wave w takes rows w, w + 4, ...; lane = seed
+    const int32_t c = cnt[lane];
+    for (int r = w; r < kTile; r += kBlock / 64) {
+      const int32_t s = s0 + r;
+      if (s < c) tile[r][lane] = slots[int64_t(s) * n + p0 + lane];
+    }
+    __syncthreads();
+    // write: element e -> seed e / kTile, slot e % kTile
+    for (int e = t; e < kTile * kTile; e += kBlock) {
+      const int q = e / kTile, j = e % kTile;
+      if (s0 + j < cnt[q]) out[dst[q] + s0 + j] = tile[j][q];
+    }
+    __syncthreads();
+  }
+}
+
 // --------------------------------------------------------- hand-off groups
 constexpr int kBuckets = 12;  // (map 0/1) x (neighbour slot 0..5)
 
@@ -364,6 +407,39 @@ pa_status pa_gpu_prune_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t str
   hipLaunchKernelGGL(k_copy_counted, dim3(unsigned(blocks)), dim3(kBlock), 0, ctx->stream,
                      reinterpret_cast<const float4*>(d_slots), int64_t(stride_slots), d_vertex_counts, offs,
                      int64_t(n), reinterpret_cast<float4*>(d_out));
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+
+pa_status pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots, size_t n,
+                                          const int32_t* d_vertex_counts, float* d_out, size_t capacity,
+                                          size_t* n_out, int64_t* d_offsets) {
+  if (!ctx || !n_out || (n && (!d_slots || !d_vertex_counts || !d_out))) return PA_E_INVALID;
+  if (n == 0) {
+    *n_out = 0;
+    return PA_OK;
+  }
+  int64_t* offs = d_offsets;
+  void* ws = nullptr;
+  const size_t own = d_offsets ? 0 : (n + 1) * sizeof(int64_t);
+  PA_TRY(scratch(ctx, own + scan_ws_total(int64_t(n)) + 4096, &ws));
+  if (!offs) offs = static_cast<int64_t*>(ws);
+  int64_t* sws = reinterpret_cast<int64_t*>(static_cast<char*>(ws) + own);
+  launch_begin(ctx, "prune_scan");
+  PA_TRY(scan_rec<int32_t>(ctx, d_vertex_counts, int64_t(n), offs, sws));
+  launch_end(ctx);
+  int64_t total = 0;
+  PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *n_out = size_t(total);
+  if (size_t(total) > capacity) return fail(ctx, PA_E_CAPACITY, "prune_counted: output capacity too small");
+  const int64_t blocks = (int64_t(n) + kTile - 1) / kTile;
+  if (blocks > INT32_MAX) return fail(ctx, PA_E_INVALID, "prune_counted: too many seeds");
+  launch_begin(ctx, "prune_counted");
+  hipLaunchKernelGGL(k_copy_counted_step_major, dim3(unsigned(blocks)), dim3(kBlock), 0, ctx->stream,
+                     reinterpret_cast<const float4*>(d_slots), int64_t(n), d_vertex_counts, offs,
+                     reinterpret_cast<float4*>(d_out));
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;

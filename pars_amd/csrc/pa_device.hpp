@@ -94,6 +94,40 @@ __device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, f
 
 __device__ __forceinline__ float lerp1(float a, float b, float w, float u) { return u * a + w * b; }
 
+// Cell index and weights of vector_field::interpolate (vector_field.cpp:
+// 27-31): m = floor(q); w = fmod(d, h) / h (reference) or q - m
+// (consistent); u = 1 - w.
+template <int INTERP>
+__device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3], const float q[3], int m[3],
+                                             float w[3], float u[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float fl = floorf(q[i]);
+    m[i] = static_cast<int>(fl);
+    if constexpr (INTERP == PA_INTERP_CONSISTENT) {
+      w[i] = q[i] - fl;
+    } else {
+      w[i] = exact_fmod(d[i], f.spacing[i], q[i]) / f.spacing[i];
+    }
+    u[i] = 1.0f - w[i];
+  }
+}
+
+// The lerp tree of vector_field.cpp:41-47 over the 8 corners
+// c[0..23] = c000 c001 c010 c011 c100 c101 c110 c111 (xyz each).
+__device__ __forceinline__ void lerp_corners(const float* c, const float w[3], const float u[3], float v[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float c00 = lerp1(c[0 + k], c[3 + k], w[2], u[2]);
+    const float c01 = lerp1(c[6 + k], c[9 + k], w[2], u[2]);
+    const float c10 = lerp1(c[12 + k], c[15 + k], w[2], u[2]);
+    const float c11 = lerp1(c[18 + k], c[21 + k], w[2], u[2]);
+    const float c0 = lerp1(c00, c01, w[1], u[1]);
+    const float c1 = lerp1(c10, c11, w[1], u[1]);
+    v[k] = lerp1(c0, c1, w[0], u[0]);
+  }
+}
+
 // vector_field::interpolate (vector_field.cpp:22-48): m = floor(q), w =
 // fmod(d, h) / h, lerp (linear_interpolate.hpp:6-10) along z, y, x.  With
 // INTERP == PA_INTERP_CONSISTENT the weight is q - floor(q) (exact in FP32
@@ -256,6 +290,18 @@ __device__ __forceinline__ int advance(const TraceConfig& c, const FieldDesc& f,
     next = step_frozen<MODE>(c, last, v);
   }
   return 0;
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on
+// XCD i % 8).  With remap on, XCD x processes the contiguous x-th eighth of
+// the logical workgroups, so particles that start close together share an
+// XCD's L2.  A bijection for any grid size.
+__device__ __forceinline__ int64_t logical_block(int remap) {
+  const int64_t b = blockIdx.x;
+  if (!remap) return b;
+  const int64_t nb = gridDim.x, q = nb / 8, r = nb % 8;
+  const int64_t x = b % 8, k = b / 8;
+  return x * q + (x < r ? x : r) + k;
 }
 
 template <int V>

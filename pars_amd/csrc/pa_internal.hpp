@@ -50,6 +50,7 @@ struct TraceConfig {
   float one_dt;              // 1 * dt
   float stop_eps2;
   int32_t interp;
+  int32_t xcd_remap;
 };
 
 }  // namespace pa_amd
@@ -93,6 +94,9 @@ struct pa_gpu_tracer {
   int32_t stop_rule = PA_STOP_ISZERO;
   float stop_eps = 0.0f;
   int32_t interp = PA_INTERP_REFERENCE;
+  int32_t engine = PA_ENGINE_LANE;
+  int32_t waves = 0;  // resident waves of the cooperative engine (0 = automatic)
+  int32_t xcd_remap = 1;
 };
 
 namespace pa_amd {
@@ -106,6 +110,26 @@ void launch_begin(pa_gpu_ctx* ctx, const char* name);
 void launch_end(pa_gpu_ctx* ctx);
 // Build the per-launch configuration from a tracer.
 pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg);
+
+// Launch of the wave-cooperative tracer (trace_coop.hip).
+struct CoopLaunch {
+  const pa_particle* particles = nullptr;
+  int64_t n = 0;
+  int32_t* counts = nullptr;
+  int stop_rule = PA_STOP_ISZERO;
+  int waves = 0;
+  float4* verts = nullptr;
+  int64_t region = 0;
+  pa_particle* cont = nullptr;
+  float4* endpoints = nullptr;
+  int32_t* seed_steps = nullptr;
+  float4* const* chunks = nullptr;
+  int64_t ppic = 0;
+  int64_t maxR = 0;
+  pa_particle* hand_tmp = nullptr;
+  int32_t* hand_code = nullptr;
+};
+pa_status launch_coop(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, bool union_form, const CoopLaunch& l);
 
 // Exclusive scan of n int64 values in place on the device (n+1 outputs:
 // out[n] = total).  Uses ctx scratch.

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TraceConfig cfg, const p
                                                   int32_t* __restrict__ counts, pa_particle* __restrict__ hand_tmp,
                                                   int32_t* __restrict__ hand_code,
                                                   unsigned long long* __restrict__ steps_out, int stop_rule) {
-  const int64_t pi = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t pi = logical_block(cfg.xcd_remap) * kBlock + threadIdx.x;
   unsigned long long steps = 0;
   if (pi < n) {
     const pa_particle p = P[pi];
@@ -160,17 +160,22 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TraceConfig cfg, const p
 template <int S, int MODE, int INTERP>
 __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
                                                         int64_t n, float4* __restrict__ verts, int64_t region,
+                                                        int64_t seed_stride, int64_t slot_stride,
                                                         int32_t* __restrict__ counts, pa_particle* __restrict__ cont,
                                                         unsigned int* __restrict__ n_cont,
                                                         unsigned long long* __restrict__ steps_out, int stop_rule,
                                                         float4* __restrict__ endpoints, int32_t* __restrict__ seed_steps) {
-  const int64_t pi = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t pi = logical_block(cfg.xcd_remap) * kBlock + threadIdx.x;
   unsigned long long steps = 0;
   if (pi < n) {
     const pa_particle p = P[pi];
-    float4* out = verts + pi * region;
+    // Slot s of seed pi lives at verts[pi * seed_stride + s * slot_stride]:
+    // seed-major (region, 1) or step-major (1, n).  Step-major makes the
+    // lanes of a wave, which step in lockstep, store one contiguous run.
+    float4* const base_ptr = verts + pi * seed_stride;
+    auto out = [&](int64_t sl) -> float4& { return base_ptr[sl * slot_stride]; };
     float4 last = make_float4(p.position[0], p.position[1], p.position[2], p.position[3]);
-    out[0] = last;
+    out(0) = last;
     int b = p.vector_field_index;
     int R = p.remaining_iterations;
     int64_t base = 0;
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, c
           exited = a == 1;
           break;
         }
-        out[base + it] = next;
+        out(base + it) = next;
         last = next;
         ++steps;
       }
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, c
         used = base + 1;
         break;
       }
-      out[base + it] = termination_vertex();
+      out(base + it) = termination_vertex();
       used = base + it + 1;
       if (!exited) break;
       int nb;
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, c
         cont[atomicAdd(n_cont, 1u)] = c;
         break;
       }
-      out[used] = last;
+      out(used) = last;
       base = used;
       R = Rn;
       b = nb;
@@ -422,6 +427,7 @@ pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg) {
   cfg->one_dt = 1.0f * dt;
   cfg->stop_eps2 = t->stop_eps * t->stop_eps;
   cfg->interp = t->interp;
+  cfg->xcd_remap = t->xcd_remap;
   return PA_OK;
 }
 
@@ -521,6 +527,21 @@ pa_status pa_gpu_tracer_set_stop_rule(pa_gpu_tracer* t, pa_stop_rule rule, float
   if (rule != PA_STOP_ISZERO && rule != PA_STOP_NORM) return fail(t->ctx, PA_E_INVALID, "unknown stop rule");
   t->stop_rule = rule;
   t->stop_eps = eps;
+  return PA_OK;
+}
+
+pa_status pa_gpu_tracer_set_engine(pa_gpu_tracer* t, pa_engine engine, int32_t waves) {
+  if (!t) return PA_E_INVALID;
+  if (engine != PA_ENGINE_LANE && engine != PA_ENGINE_COOPERATIVE) return fail(t->ctx, PA_E_INVALID, "unknown engine");
+  if (waves < 0) return fail(t->ctx, PA_E_INVALID, "waves must be >= 0");
+  t->engine = engine;
+  t->waves = waves;
+  return PA_OK;
+}
+
+pa_status pa_gpu_tracer_set_xcd_remap(pa_gpu_tracer* t, int32_t enable) {
+  if (!t) return PA_E_INVALID;
+  t->xcd_remap = enable ? 1 : 0;
   return PA_OK;
 }
 
@@ -636,6 +657,24 @@ pa_status pa_gpu_tracer_trace(pa_gpu_tracer* t, const pa_particle* d_particles, 
   auto* steps = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 1024);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 1);
   const int mode = mode_of(t->integrator);
+  if (t->engine == PA_ENGINE_COOPERATIVE) {
+    CoopLaunch L;
+    L.particles = d_particles;
+    L.n = int64_t(n);
+    L.counts = d_vertex_counts;
+    L.stop_rule = t->stop_rule;
+    L.waves = t->waves;
+    L.chunks = reinterpret_cast<float4* const*>(w);
+    L.ppic = int64_t(ppic);
+    L.maxR = int64_t(maxR);
+    L.hand_tmp = tmp;
+    L.hand_code = code;
+    if (!d_vertex_counts) {
+      // the cooperative engine always records counts; keep them in scratch
+      return fail(ctx, PA_E_INVALID, "the cooperative engine needs d_vertex_counts");
+    }
+    PA_TRY(launch_coop(ctx, cfg, mode, false, L));
+  } else {
   launch_begin(ctx, "trace");
   dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
     hipLaunchKernelGGL((k_trace<decltype(S)::value, decltype(M)::value, decltype(I)::value>), dim3(grid_of(int64_t(n))),
@@ -644,6 +683,7 @@ pa_status pa_gpu_tracer_trace(pa_gpu_tracer* t, const pa_particle* d_particles, 
                        steps, t->stop_rule);
   }, cfg.field[0].layout);
   launch_end(ctx);
+  }
   PA_HIP(ctx, hipGetLastError());
   if (handoff) PA_TRY(group_handoffs(ctx, code, tmp, int64_t(n), handoff, w + table + tmp_bytes));
   unsigned long long s = 0;
@@ -657,6 +697,16 @@ pa_status pa_gpu_tracer_trace_union(pa_gpu_tracer* t, const pa_particle* d_parti
                                     float* d_vertices, int32_t* d_vertex_counts, pa_particle* d_continuation,
                                     size_t* n_continuation, float* d_endpoints, int32_t* d_seed_steps,
                                     uint64_t* total_steps) {
+  return pa_gpu_tracer_trace_union_layout(t, d_particles, n, region_slots, PA_SLOTS_SEED_MAJOR, d_vertices,
+                                          d_vertex_counts, d_continuation, n_continuation, d_endpoints, d_seed_steps,
+                                          total_steps);
+}
+
+pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n,
+                                           size_t region_slots, pa_slot_order order, float* d_vertices,
+                                           int32_t* d_vertex_counts, pa_particle* d_continuation,
+                                           size_t* n_continuation, float* d_endpoints, int32_t* d_seed_steps,
+                                           uint64_t* total_steps) {
   if (!t) return PA_E_INVALID;
   pa_gpu_ctx* ctx = t->ctx;
   if (n_continuation) *n_continuation = 0;
@@ -673,19 +723,41 @@ pa_status pa_gpu_tracer_trace_union(pa_gpu_tracer* t, const pa_particle* d_parti
   if (maxr < 0) return fail(ctx, PA_E_INVALID, "negative remaining_iterations");
   if (region_slots < size_t(maxr < 1 ? 1 : maxr))
     return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
+  if (order != PA_SLOTS_SEED_MAJOR && order != PA_SLOTS_STEP_MAJOR)
+    return fail(ctx, PA_E_INVALID, "unknown slot order");
+  if (order == PA_SLOTS_STEP_MAJOR && t->engine != PA_ENGINE_LANE)
+    return fail(ctx, PA_E_UNSUPPORTED, "step-major slots need the lane engine");
+  const int64_t seed_stride = order == PA_SLOTS_STEP_MAJOR ? 1 : int64_t(region_slots);
+  const int64_t slot_stride = order == PA_SLOTS_STEP_MAJOR ? int64_t(n) : 1;
   char* cbase = static_cast<char*>(ctx->counters);
   auto* steps = reinterpret_cast<unsigned long long*>(cbase + 1024);
   auto* ncont = reinterpret_cast<unsigned*>(cbase + 1032);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 2);
   const int mode = mode_of(t->integrator);
+  if (t->engine == PA_ENGINE_COOPERATIVE) {
+    CoopLaunch L;
+    L.particles = d_particles;
+    L.n = int64_t(n);
+    L.counts = d_vertex_counts;
+    L.stop_rule = t->stop_rule;
+    L.waves = t->waves;
+    L.verts = reinterpret_cast<float4*>(d_vertices);
+    L.region = int64_t(region_slots);
+    L.cont = d_continuation;
+    L.endpoints = reinterpret_cast<float4*>(d_endpoints);
+    L.seed_steps = d_seed_steps;
+    PA_TRY(launch_coop(ctx, cfg, mode, true, L));
+  } else {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 2);
   launch_begin(ctx, "trace_union");
   dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
     hipLaunchKernelGGL((k_trace_union<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
                        dim3(grid_of(int64_t(n))), dim3(kBlock), 0, ctx->stream, cfg, d_particles, int64_t(n),
-                       reinterpret_cast<float4*>(d_vertices), int64_t(region_slots), d_vertex_counts, d_continuation,
-                       ncont, steps, t->stop_rule, reinterpret_cast<float4*>(d_endpoints), d_seed_steps);
+                       reinterpret_cast<float4*>(d_vertices), int64_t(region_slots), seed_stride, slot_stride,
+                       d_vertex_counts, d_continuation, ncont, steps, t->stop_rule,
+                       reinterpret_cast<float4*>(d_endpoints), d_seed_steps);
   }, cfg.field[0].layout);
   launch_end(ctx);
+  }
   PA_HIP(ctx, hipGetLastError());
   unsigned long long h[2] = {0, 0};
   PA_HIP(ctx, hipMemcpyAsync(h, steps, 16, hipMemcpyDeviceToHost, ctx->stream));

@@ -41,8 +41,10 @@ class ShareFields:
     def neighbors(self):
         return self.dev[1:]
 
-    def tracer(self, ctx, dt, integrator=pa.RUNGE_KUTTA_4, interp=pa.INTERP_REFERENCE):
+    def tracer(self, ctx, dt, integrator=pa.RUNGE_KUTTA_4, interp=pa.INTERP_REFERENCE, engine=pa.ENGINE_COOPERATIVE,
+               waves=0):
         t = pa.particle_tracer(ctx)
+        t.set_engine(engine, waves)
         t.set_local_vector_field(self.local)
         t.set_neighbor_vector_fields(self.neighbors, self.share.neighbor_ranks)
         t.set_neighbor_face_masks(self.share.face_masks)

@@ -108,14 +108,15 @@ def test_trace_config1_bitexact(ctx, orc):
 @pytest.mark.parametrize("integ", [pa.RUNGE_KUTTA_4, pa.EULER, pa.CLASSIC_RK4])
 @pytest.mark.parametrize("interp", [pa.INTERP_REFERENCE, pa.INTERP_CONSISTENT])
 @pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
-def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout):
+@pytest.mark.parametrize("engine", [pa.ENGINE_COOPERATIVE, pa.ENGINE_LANE])
+def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout, engine):
     """One rank's share (own block + 6 neighbours, 4x4x4 grid of a 64^3 ABC
     domain, rank 21), own-block seeds plus load-balanced particles that live in
     neighbour blocks; slots, counts and both hand-off maps compared."""
     S = ShareFields(ctx, orc, 64, 64, 21, storage, layout=layout)
     sh = S.share
     dt = 0.02
-    t = S.tracer(ctx, dt, integ, interp)
+    t = S.tracer(ctx, dt, integ, interp, engine, waves=7 if engine == pa.ENGINE_COOPERATIVE else 0)
     own = orc.seed_generate(sh.local.offset, sh.local.size, sh.local.spacing * f32(2), 150, 21)
     extra = []
     for k, nb in enumerate(sh.neighbors):
@@ -141,15 +142,20 @@ def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout):
 @pytest.mark.parametrize("storage", [pa.FP32, pa.BF16])
 @pytest.mark.parametrize("region_extra", [64, 1])
 @pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
-def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout):
+@pytest.mark.parametrize("engine,order", [(pa.ENGINE_COOPERATIVE, "seed"), (pa.ENGINE_LANE, "seed"),
+                                          (pa.ENGINE_LANE, "step")])
+@pytest.mark.parametrize("integ", [pa.RUNGE_KUTTA_4, pa.CLASSIC_RK4])
+def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, order, integ):
     """Union-of-7 continuation vs the oracle's round emulation: every seed's
-    segments, in order; a tiny region forces the continuation path."""
+    segments, in order; a tiny region forces the continuation path.  Each
+    pass is also packed by the counted prune of its slot order."""
     S = ShareFields(ctx, orc, 64, 64, 21, storage, layout=layout)
     sh = S.share
     dt, R = 0.02, 400
-    t = S.tracer(ctx, dt)
+    t = S.tracer(ctx, dt, integ, pa.INTERP_REFERENCE, engine, waves=3 if engine == pa.ENGINE_COOPERATIVE else 0)
     seeds = orc.seed_generate(sh.local.offset, sh.local.size, sh.local.spacing * f32(2), R, 21)
-    ou = orc.trace_union(seeds, S.orc[0], S.orc[1:], sh.neighbor_ranks, sh.face_masks, dt=dt, integrator=orc.RK4)
+    ou = orc.trace_union(seeds, S.orc[0], S.orc[1:], sh.neighbor_ranks, sh.face_masks, dt=dt,
+                         integrator=oracle_integrator(orc, integ))
     region = R + region_extra
     n = len(seeds)
     pending = pa.Particles.from_host(ctx, seeds)
@@ -157,12 +163,24 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout):
     got = {i: [] for i in range(n)}
     steps = 0
     for _ in range(64):
-        r = t.trace_union(pending, region)
+        r = t.trace_union(pending, region, order=order)
         steps += r["total_steps"]
         cnt = r["counts"].download(np.int32, pending.n)
-        v = r["vertices"].download(np.float32, pending.n * region * 4).reshape(pending.n, region, 4)
+        v = r["vertices"].download(np.float32, pending.n * region * 4)
+        v = v.reshape(region, pending.n, 4).transpose(1, 0, 2) if order == "step" else v.reshape(pending.n, region, 4)
         for j in range(pending.n):
             got[int(seed_of[j])].append(v[j, :cnt[j]])
+        total = int(cnt.sum())
+        if order == "step":
+            packed, m, offs = pa.particle_tracer.prune_counted_step_major(ctx, r["vertices"].ptr, r["counts"],
+                                                                          pending.n, total)
+        else:
+            packed, m, offs = pa.particle_tracer.prune_counted(ctx, r["vertices"].ptr, region, r["counts"],
+                                                               pending.n, total)
+        assert m == total
+        want_packed = np.concatenate([v[j, :cnt[j]] for j in range(pending.n)])
+        assert packed.download(np.float32, 4 * m).tobytes() == want_packed.tobytes()
+        assert np.array_equal(offs.download(np.int64, pending.n + 1), np.concatenate([[0], np.cumsum(cnt)]))
         cont = r["continuation"]
         if cont.n == 0:
             break
