@@ -59,8 +59,12 @@ class Pipeline:
         t.set_step_size(DT)
         t.set_stop_rule(pa.STOP_NORM, 1e-6)
         t.set_interpolation(interp)
+        if order == "chunked":
+            t.set_engine(pa.ENGINE_QUAD)
         self.tracer = t
         self.region = R + EXTRA_SLOTS
+        if order == "chunked":
+            self.region = (self.region + 7) // 8 * 8
         self.seeds = ctx.alloc(n_max * 48)
         self.ub = t.union_buffers(n_max, self.region, endpoints=True)
         self.cap = n_max * self.region
@@ -83,7 +87,12 @@ class Pipeline:
         if r["continuation"].n:
             raise RuntimeError("segment region overflow: %d seeds need a continuation pass" % r["continuation"].n)
         m = C.c_size_t()
-        if self.order == "step":
+        if self.order == "chunked":
+            ctx.check(lib.pa_gpu_prune_counted_chunked(
+                ctx.handle, C.c_void_p(self.ub["vertices"].ptr), C.c_size_t(self.region),
+                C.c_void_p(self.ub["counts"].ptr), n, C.c_void_p(self.pruned.ptr), C.c_size_t(self.cap),
+                C.byref(m), C.c_void_p(self.offs.ptr)), "prune")
+        elif self.order == "step":
             ctx.check(lib.pa_gpu_prune_counted_step_major(
                 ctx.handle, C.c_void_p(self.ub["vertices"].ptr), n, C.c_void_p(self.ub["counts"].ptr),
                 C.c_void_p(self.pruned.ptr), C.c_size_t(self.cap), C.byref(m), C.c_void_p(self.offs.ptr)), "prune")
@@ -227,7 +236,8 @@ def main():
             log("fields %s: %.1f MB in %.2f s" % (form, sum(f.nbytes() for f in fields[key]) / 1e6,
                                                   time.perf_counter() - t0))
         interp = pa.INTERP_CONSISTENT if "consistent" in form.split("-") else pa.INTERP_REFERENCE
-        order = "seed" if "seedmajor" in form.split("-") else "step"
+        tokens = form.split("-")
+        order = "seed" if "seedmajor" in tokens else "chunked" if "quad" in tokens else "step"
         p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order)
         log("%s: %.3f ms/step, %.3e steps/s, trace kernel %.3f ms, %s" %
             (form, res["ms_per_step"], res["value"], res["trace_ms"], res["info"]))

@@ -238,10 +238,13 @@ pa_status   pa_gpu_tracer_set_interpolation(pa_gpu_tracer* tracer, pa_interp int
  * particle for the arithmetic but gathers the 8 corners cooperatively (groups
  * of lanes load one particle's cell record or z-pairs as coalesced 16-B
  * pieces, redistributed through LDS) and refills finished lanes from a work
- * counter.  Both produce identical results; PA_ENGINE_LANE is the default
- * (faster on gfx950).  waves = resident waves of the cooperative engine (0 =
- * automatic). */
-typedef enum pa_engine { PA_ENGINE_LANE = 0, PA_ENGINE_COOPERATIVE = 1 } pa_engine;
+ * counter.  PA_ENGINE_QUAD (union trace only, chunked slots) splits every
+ * particle over the four lanes of a quad, one vec4 component each, which
+ * shortens the serial step chain that the longest trajectories of a round
+ * sit on; persistent waves refill quads from a work counter.  All engines
+ * produce identical results; PA_ENGINE_LANE is the default.  waves =
+ * resident waves of the cooperative engine (0 = automatic). */
+typedef enum pa_engine { PA_ENGINE_LANE = 0, PA_ENGINE_COOPERATIVE = 1, PA_ENGINE_QUAD = 2 } pa_engine;
 pa_status   pa_gpu_tracer_set_engine(pa_gpu_tracer* tracer, pa_engine engine, int32_t waves);
 /* Extension: XCD-aware workgroup order of the lane engine (default on):
  * the 8 XCDs, which take workgroups round-robin, each process a contiguous
@@ -298,8 +301,16 @@ pa_status   pa_gpu_tracer_trace_union(pa_gpu_tracer* tracer, const pa_particle* 
  * lanes of a wave step in lockstep, so every vertex store is one contiguous
  * run, which is what the memory system rewards.  Pack either into the
  * reference's pruned layout with pa_gpu_prune_counted /
- * pa_gpu_prune_counted_step_major.  Step-major needs PA_ENGINE_LANE. */
-typedef enum pa_slot_order { PA_SLOTS_SEED_MAJOR = 0, PA_SLOTS_STEP_MAJOR = 1 } pa_slot_order;
+ * pa_gpu_prune_counted_step_major / pa_gpu_prune_counted_chunked.
+ * Step-major needs PA_ENGINE_LANE, chunked PA_ENGINE_QUAD. */
+typedef enum pa_slot_order {
+  PA_SLOTS_SEED_MAJOR = 0,
+  PA_SLOTS_STEP_MAJOR = 1,
+  /* seed-major 8-slot chunks, component-planar: slot s of seed p is float
+   * (p * region_slots + s - s % 8) * 4 + 8 * c + s % 8 for component c;
+   * region_slots a multiple of 8; written by PA_ENGINE_QUAD only. */
+  PA_SLOTS_CHUNKED = 2
+} pa_slot_order;
 pa_status   pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
                                              size_t region_slots, pa_slot_order order, float* d_vertices,
                                              int32_t* d_vertex_counts, pa_particle* d_continuation,
@@ -310,6 +321,10 @@ pa_status   pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* tracer, const pa_par
 pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots, size_t n,
                                             const int32_t* d_vertex_counts, float* d_out, size_t capacity,
                                             size_t* n_out, int64_t* d_offsets);
+/* Counted prune of chunked slots (PA_SLOTS_CHUNKED) into the pruned layout. */
+pa_status   pa_gpu_prune_counted_chunked(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
+                                         const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,
+                                         size_t* n_out, int64_t* d_offsets);
 
 /* prune (particle_tracer.cpp:402-409): order-preserving removal of
  * invalid_vertex from one chunk.  *n_out receives the kept count. */

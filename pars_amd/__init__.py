@@ -51,9 +51,9 @@ INTEGRATORS = {
 STOP_ISZERO, STOP_NORM = 0, 1
 INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
 LAYOUT_NODES, LAYOUT_CELLS = 0, 1
-ENGINE_LANE, ENGINE_COOPERATIVE = 0, 1
-SLOTS_SEED_MAJOR, SLOTS_STEP_MAJOR = 0, 1
-SLOT_ORDERS = {"seed": SLOTS_SEED_MAJOR, "step": SLOTS_STEP_MAJOR}
+ENGINE_LANE, ENGINE_COOPERATIVE, ENGINE_QUAD = 0, 1, 2
+SLOTS_SEED_MAJOR, SLOTS_STEP_MAJOR, SLOTS_CHUNKED = 0, 1, 2
+SLOT_ORDERS = {"seed": SLOTS_SEED_MAJOR, "step": SLOTS_STEP_MAJOR, "chunked": SLOTS_CHUNKED}
 TERMINATION_VERTEX = np.array([-2.0, -2.0, -2.0, 0.0], np.float32)
 INVALID_VERTEX = np.array([-1.0, -1.0, -1.0, 0.0], np.float32)
 INVALID_INDEX = -1
@@ -386,7 +386,8 @@ class particle_tracer:
         self.ctx.check(lib().pa_gpu_tracer_set_step_size(self._h, C.c_float(step_size)))
 
     def set_engine(self, engine: int, waves: int = 0):
-        """ENGINE_COOPERATIVE (default) or ENGINE_LANE; waves = resident waves (0 = automatic)."""
+        """ENGINE_LANE (default), ENGINE_COOPERATIVE or ENGINE_QUAD (union trace, chunked slots);
+        waves = resident waves of the cooperative engine (0 = automatic)."""
         self.ctx.check(lib().pa_gpu_tracer_set_engine(self._h, C.c_int(engine), C.c_int32(waves)), "set_engine")
 
     def set_xcd_remap(self, enable: bool):
@@ -443,7 +444,8 @@ class particle_tracer:
     def trace_union(self, particles: Particles, region_slots: int, buffers=None, endpoints=False, order="seed"):
         """Union-of-7 trace; buffers (from union_buffers) may be reused across calls.
 
-        order "seed": slot s of seed p at p * region_slots + s; "step": at s * n + p."""
+        order "seed": slot s of seed p at p * region_slots + s; "step": at s * n + p;
+        "chunked" (quad engine): 8-slot component-planar chunks per seed."""
         b = buffers or self.union_buffers(particles.n, region_slots, endpoints)
         ncont = C.c_size_t()
         steps = C.c_uint64()
@@ -492,6 +494,18 @@ class particle_tracer:
         ctx.check(lib().pa_gpu_prune_counted(ctx.handle, C.c_void_p(slots), C.c_size_t(stride_slots),
                                              C.c_void_p(counts.ptr), C.c_size_t(n), C.c_void_p(out.ptr),
                                              C.c_size_t(capacity), C.byref(m), C.c_void_p(offs.ptr)), "prune_counted")
+        return out, m.value, offs
+
+    @staticmethod
+    def prune_counted_chunked(ctx: Context, slots: int, region_slots: int, counts: DeviceBuffer, n: int,
+                              capacity: int, out: DeviceBuffer = None, offs: DeviceBuffer = None):
+        """Pack chunked union slots (quad engine) into the pruned particle-major layout."""
+        out = ctx.alloc(max(capacity, 1) * 16) if out is None else out
+        offs = ctx.alloc((n + 1) * 8) if offs is None else offs
+        m = C.c_size_t()
+        ctx.check(lib().pa_gpu_prune_counted_chunked(
+            ctx.handle, C.c_void_p(slots), C.c_size_t(region_slots), C.c_void_p(counts.ptr), C.c_size_t(n),
+            C.c_void_p(out.ptr), C.c_size_t(capacity), C.byref(m), C.c_void_p(offs.ptr)), "prune_counted_chunked")
         return out, m.value, offs
 
     @staticmethod

@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libpa_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PA_GPU_ARCH", "gfx950")
 
-SOURCES = ["ctx.hip", "field.hip", "trace.hip", "trace_coop.hip", "compact.hip", "color.hip", "ftle.hip"]
+SOURCES = ["ctx.hip", "field.hip", "trace.hip", "trace_coop.hip", "trace_quad.hip", "compact.hip", "color.hip",
+           "ftle.hip"]
 
 FLAGS = [
     "--offload-arch=" + ARCH,

@@ -261,6 +261,25 @@ __global__ __launch_bounds__(kBlock) void k_copy_counted_step_major(const float4
   }
 }
 
+// Chunked slots (8-slot component-planar chunks per seed) into the pruned
+// layout: one wave per seed, lane = vertex; the four component loads of a
+// wave hit the same few lines and every store is a contiguous 1 KiB run.
+__global__ __launch_bounds__(kBlock) void k_copy_counted_chunked(const float* __restrict__ slots, int64_t region,
+                                                                 const int32_t* __restrict__ counts,
+                                                                 const int64_t* __restrict__ offs, int64_t n,
+                                                                 float4* __restrict__ out) {
+  const int64_t waves = int64_t(gridDim.x) * (kBlock / 64);
+  for (int64_t p = int64_t(blockIdx.x) * (kBlock / 64) + wave_id(); p < n; p += waves) {
+    const int32_t c = counts[p];
+    const float* src = slots + p * region * 4;
+    float4* dst = out + offs[p];
+    for (int j = lane_id(); j < c; j += 64) {
+      const float* ch = src + int64_t(j >> 3) * 32 + (j & 7);
+      dst[j] = make_float4(ch[0], ch[8], ch[16], ch[24]);
+    }
+  }
+}
+
 // --------------------------------------------------------- hand-off groups
 constexpr int kBuckets = 12;  // (map 0/1) x (neighbour slot 0..5)
 
@@ -407,6 +426,39 @@ pa_status pa_gpu_prune_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t str
   hipLaunchKernelGGL(k_copy_counted, dim3(unsigned(blocks)), dim3(kBlock), 0, ctx->stream,
                      reinterpret_cast<const float4*>(d_slots), int64_t(stride_slots), d_vertex_counts, offs,
                      int64_t(n), reinterpret_cast<float4*>(d_out));
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+
+pa_status pa_gpu_prune_counted_chunked(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
+                                       const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,
+                                       size_t* n_out, int64_t* d_offsets) {
+  if (!ctx || !n_out || (n && (!d_slots || !d_vertex_counts || !d_out))) return PA_E_INVALID;
+  if (region_slots % 8 != 0) return fail(ctx, PA_E_INVALID, "chunked slots: region_slots must be a multiple of 8");
+  if (n == 0) {
+    *n_out = 0;
+    return PA_OK;
+  }
+  int64_t* offs = d_offsets;
+  void* ws = nullptr;
+  const size_t own = d_offsets ? 0 : (n + 1) * sizeof(int64_t);
+  PA_TRY(scratch(ctx, own + scan_ws_total(int64_t(n)) + 4096, &ws));
+  if (!offs) offs = static_cast<int64_t*>(ws);
+  int64_t* sws = reinterpret_cast<int64_t*>(static_cast<char*>(ws) + own);
+  launch_begin(ctx, "prune_scan");
+  PA_TRY(scan_rec<int32_t>(ctx, d_vertex_counts, int64_t(n), offs, sws));
+  launch_end(ctx);
+  int64_t total = 0;
+  PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *n_out = size_t(total);
+  if (size_t(total) > capacity) return fail(ctx, PA_E_CAPACITY, "prune_counted: output capacity too small");
+  int64_t blocks = (int64_t(n) + 3) / 4;
+  if (blocks > 65536) blocks = 65536;
+  launch_begin(ctx, "prune_counted");
+  hipLaunchKernelGGL(k_copy_counted_chunked, dim3(unsigned(blocks)), dim3(kBlock), 0, ctx->stream, d_slots,
+                     int64_t(region_slots), d_vertex_counts, offs, int64_t(n), reinterpret_cast<float4*>(d_out));
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;

@@ -100,14 +100,16 @@ __global__ void k_cells(const void* __restrict__ nodes, void* __restrict__ cells
     const int64_t n0 = i * ns0 + j * ns1 + k;
     const int64_t corner[8] = {n0, n0 + 1, n0 + ns1, n0 + ns1 + 1, n0 + ns0, n0 + ns0 + 1, n0 + ns0 + ns1,
                                n0 + ns0 + ns1 + 1};
+    // Record = x of the 8 corners, then y, then z, then padding (one
+    // component of all corners is one contiguous quarter of the record).
     if (storage == PA_STORAGE_FP32) {
       float r[32];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const float4 v = static_cast<const float4*>(nodes)[corner[q]];
-        r[3 * q] = v.x;
-        r[3 * q + 1] = v.y;
-        r[3 * q + 2] = v.z;
+        r[q] = v.x;
+        r[8 + q] = v.y;
+        r[16 + q] = v.z;
       }
 #pragma unroll
       for (int q = 24; q < 32; ++q) r[q] = 0.0f;
@@ -119,9 +121,9 @@ __global__ void k_cells(const void* __restrict__ nodes, void* __restrict__ cells
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const uint2 u = static_cast<const uint2*>(nodes)[corner[q]];
-        hw[3 * q] = u.x & 0xffffu;
-        hw[3 * q + 1] = u.x >> 16;
-        hw[3 * q + 2] = u.y & 0xffffu;
+        hw[q] = u.x & 0xffffu;
+        hw[8 + q] = u.x >> 16;
+        hw[16 + q] = u.y & 0xffffu;
       }
 #pragma unroll
       for (int q = 24; q < 32; ++q) hw[q] = 0u;
@@ -151,6 +153,10 @@ pa_status make_field(pa_gpu_ctx* ctx, const int32_t dims[3], const float offset[
     if (!(spacing[i] > 0.0f) || !std::isfinite(spacing[i]))
       return fail(ctx, PA_E_INVALID, "spacing must be positive and finite");
   }
+  for (int i = 0; i < 3; ++i)
+    if (dims[i] > (1 << 22)) return fail(ctx, PA_E_INVALID, "a block axis holds at most 2^22 nodes");
+  if (int64_t(dims[0]) * dims[1] * dims[2] > int64_t(UINT32_MAX))
+    return fail(ctx, PA_E_INVALID, "a block holds fewer than 2^32 nodes");
   auto* f = new pa_gpu_field();
   const int64_t n = int64_t(dims[0]) * dims[1] * dims[2];
   const size_t node_bytes = storage == PA_STORAGE_FP32 ? 16 : 8;
@@ -169,7 +175,11 @@ pa_status make_field(pa_gpu_ctx* ctx, const int32_t dims[3], const float offset[
     d.spacing[i] = spacing[i];
     d.size[i] = size[i];
     d.upper[i] = offset[i] + size[i];  // particle_tracer.cpp:263, one FP32 add
+    d.inv_spacing[i] = 1.0f / spacing[i];
   }
+  d.fast_div = 1;
+  for (int i = 0; i < 3; ++i)
+    if (!(spacing[i] >= 0x1p-20f && spacing[i] <= 0x1p20f)) d.fast_div = 0;
   d.stride1 = dims[2];
   d.stride0 = int64_t(dims[1]) * dims[2];
   d.cells = nullptr;
@@ -267,6 +277,7 @@ pa_status pa_gpu_field_set_layout(pa_gpu_ctx* ctx, pa_gpu_field* field, pa_layou
   if (layout != PA_LAYOUT_CELLS) return fail(ctx, PA_E_INVALID, "unknown layout");
   if (!field->d_cells) {
     const int64_t ncells = int64_t(d.dims[0] - 1) * (d.dims[1] - 1) * (d.dims[2] - 1);
+    if (ncells >= (int64_t(1) << 31)) return fail(ctx, PA_E_INVALID, "cell records: a block holds fewer than 2^31 cells");
     field->cell_bytes = size_t(ncells) * (d.storage == PA_STORAGE_FP32 ? 128 : 64);
     PA_HIP(ctx, hipMalloc(&field->d_cells, field->cell_bytes));
     launch_begin(ctx, "field_cells");

@@ -27,6 +27,17 @@ __device__ __forceinline__ float f16_hi(uint32_t u) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(u >> 16)));
 }
 
+// Element i of a global-memory array whose pointer the compiler cannot
+// place (e.g. read from LDS): a global load rather than a flat one.
+template <class T>
+__device__ __forceinline__ T gload(const void* base, uint64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(1))) T*)(base))[i];
+#else
+  return static_cast<const T*>(base)[i];
+#endif
+}
+
 // Template storage code S = storage + 3 * layout.
 constexpr int storage_of(int S) { return S % 3; }
 constexpr int layout_of(int S) { return S / 3; }
@@ -39,7 +50,7 @@ __device__ __forceinline__ float dec16(uint32_t h, int storage) {
 // One node of a block, decoded to FP32 (x, y, z); the 4th component is
 // padding.
 template <int S>
-__device__ __forceinline__ void load_node(const void* base, int64_t i, float& x, float& y, float& z) {
+__device__ __forceinline__ void load_node(const void* base, uint64_t i, float& x, float& y, float& z) {
   if constexpr (S == PA_STORAGE_FP32) {
     const float4 v = static_cast<const float4*>(base)[i];
     x = v.x; y = v.y; z = v.z;
@@ -53,25 +64,38 @@ __device__ __forceinline__ void load_node(const void* base, int64_t i, float& x,
   }
 }
 
-// Exact fmodf(d, h) for h > 0, given q = d / h (correctly rounded).  The
-// remainder d - n*h is exactly representable once n = floor(|d|/h) is the
-// true quotient; n from floor(|q|) is off by at most one for |q| < 2^22 and
-// the sign of the FMA residual says which way.  Large quotients use the
-// library fmodf (also exact).
+// Correctly rounded a / h from y = RN(1/h) without the division sequence.
+// q0 = RN(a y) is within 1.5 ulp of a/h; one FMA correction brings it within
+// one ulp, and a second is then RN(a/h) by Markstein's theorem (the residual
+// a - q h of a quotient within one ulp is exact, and RN(q + r y) with y the
+// correctly rounded reciprocal is the correctly rounded quotient; P.
+// Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, 2nd ed., sec. 4.7).  It needs no intermediate
+// underflow or overflow: the callers take it only for a in {0} or
+// [2^-40, 2^40] with h in [2^-20, 2^20].  The sign of a zero quotient is
+// restored explicitly (h > 0).
+__device__ __forceinline__ float div_corrected(float a, float h, float y) {
+  const float q0 = a * y;
+  const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, h, a), y, q0);
+  const float q2 = __builtin_fmaf(__builtin_fmaf(-q1, h, a), y, q1);
+  return __builtin_copysignf(q2, a);
+}
+
+__device__ __forceinline__ bool div_range(float a) {
+  const float x = fabsf(a);
+  return (x == 0.0f) | ((x >= 0x1p-40f) & (x <= 0x1p40f));
+}
+
+// Exact fmodf(d, h) for h > 0 and a located point (0 <= q = d/h rounded <
+// 2^22, so |d| < 2^22 h): n = floor(q) is the true integer quotient or off
+// by one, the FMA residual d - n h is exact, and its sign (or its reaching
+// h) says which neighbour of n is the true one.  Branch-free.
 __device__ __forceinline__ float exact_fmod(float d, float h, float q) {
   const float ax = fabsf(d);
-  const float aq = fabsf(q);
-  if (!(aq < 4194304.0f)) return fmodf(d, h);
-  float n = floorf(aq);
-  float r = __builtin_fmaf(-n, h, ax);
-  if (r < 0.0f) {
-    n -= 1.0f;
-    r = __builtin_fmaf(-n, h, ax);
-  } else if (r >= h) {
-    n += 1.0f;
-    r = __builtin_fmaf(-n, h, ax);
-  }
-  return __builtin_copysignf(r, d);
+  float n = floorf(fabsf(q));
+  const float r = __builtin_fmaf(-n, h, ax);
+  n += static_cast<float>(static_cast<int>(r >= h) - static_cast<int>(r < 0.0f));
+  return __builtin_copysignf(__builtin_fmaf(-n, h, ax), d);
 }
 
 // vector_field::contains (vector_field.cpp:12-21) sharing the quotients
@@ -82,10 +106,17 @@ __device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, f
   d[0] = px - f.offset[0];
   d[1] = py - f.offset[1];
   d[2] = pz - f.offset[2];
+  const bool fast = (f.fast_div != 0) & div_range(d[0]) & div_range(d[1]) & div_range(d[2]);
+  if (__builtin_expect(fast, 1)) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) q[i] = div_corrected(d[i], f.spacing[i], f.inv_spacing[i]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) q[i] = d[i] / f.spacing[i];
+  }
   bool inside = true;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    q[i] = d[i] / f.spacing[i];
     const float s = floorf(q[i]);
     inside = inside && (s >= 0.0f) && (s <= static_cast<float>(f.dims[i] - 2));
   }
@@ -100,6 +131,8 @@ __device__ __forceinline__ float lerp1(float a, float b, float w, float u) { ret
 template <int INTERP>
 __device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3], const float q[3], int m[3],
                                              float w[3], float u[3]) {
+  float r[3];
+  bool fast = true;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float fl = floorf(q[i]);
@@ -107,10 +140,21 @@ __device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3
     if constexpr (INTERP == PA_INTERP_CONSISTENT) {
       w[i] = q[i] - fl;
     } else {
-      w[i] = exact_fmod(d[i], f.spacing[i], q[i]) / f.spacing[i];
+      r[i] = exact_fmod(d[i], f.spacing[i], q[i]);
+      fast = fast & div_range(r[i]);
     }
-    u[i] = 1.0f - w[i];
   }
+  if constexpr (INTERP != PA_INTERP_CONSISTENT) {
+    if (__builtin_expect(fast & (f.fast_div != 0), 1)) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) w[i] = div_corrected(r[i], f.spacing[i], f.inv_spacing[i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) w[i] = r[i] / f.spacing[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) u[i] = 1.0f - w[i];
 }
 
 // The lerp tree of vector_field.cpp:41-47 over the 8 corners
@@ -134,50 +178,43 @@ __device__ __forceinline__ void lerp_corners(const float* c, const float w[3], c
 // for 0 <= q < 2^23).
 template <int S, int INTERP>
 __device__ __forceinline__ void interpolate(const FieldDesc& f, const float d[3], const float q[3], float v[3]) {
+  // Cell subscripts first so the gathers are in flight while the weights
+  // are computed.
   int m[3];
-  float w[3], u[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float fl = floorf(q[i]);
-    m[i] = static_cast<int>(fl);
-    if constexpr (INTERP == PA_INTERP_CONSISTENT) {
-      w[i] = q[i] - fl;
-    } else {
-      w[i] = exact_fmod(d[i], f.spacing[i], q[i]) / f.spacing[i];
-    }
-    u[i] = 1.0f - w[i];
-  }
+  for (int i = 0; i < 3; ++i) m[i] = static_cast<int>(floorf(q[i]));
   float a0, a1, a2, b0, b1, b2, c0, c1, c2, e0, e1, e2;
   float g0, g1, g2, h0, h1, h2, k0, k1, k2, l0, l1, l2;
   if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
-    // One record holds c000 c001 c010 c011 c100 c101 c110 c111 (xyz each).
-    const int64_t ci = static_cast<int64_t>(m[0]) * f.cstride0 + static_cast<int64_t>(m[1]) * f.cstride1 + m[2];
+    // One record holds x of c000 c001 c010 c011 c100 c101 c110 c111, then
+    // y, then z.
+    // fewer than 2^31 records (checked when the layout is built)
+    const uint32_t ci = uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
     if constexpr (storage_of(S) == PA_STORAGE_FP32) {
-      const float4* r = static_cast<const float4*>(f.cells) + ci * 8;
+      const float4* r = static_cast<const float4*>(f.cells) + uint64_t(ci) * 8;
       const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
-      a0 = r0.x; a1 = r0.y; a2 = r0.z; b0 = r0.w; b1 = r1.x; b2 = r1.y;
-      c0 = r1.z; c1 = r1.w; c2 = r2.x; e0 = r2.y; e1 = r2.z; e2 = r2.w;
-      g0 = r3.x; g1 = r3.y; g2 = r3.z; h0 = r3.w; h1 = r4.x; h2 = r4.y;
-      k0 = r4.z; k1 = r4.w; k2 = r5.x; l0 = r5.y; l1 = r5.z; l2 = r5.w;
+      a0 = r0.x; b0 = r0.y; c0 = r0.z; e0 = r0.w; g0 = r1.x; h0 = r1.y; k0 = r1.z; l0 = r1.w;
+      a1 = r2.x; b1 = r2.y; c1 = r2.z; e1 = r2.w; g1 = r3.x; h1 = r3.y; k1 = r3.z; l1 = r3.w;
+      a2 = r4.x; b2 = r4.y; c2 = r4.z; e2 = r4.w; g2 = r5.x; h2 = r5.y; k2 = r5.z; l2 = r5.w;
     } else {
       constexpr int st = storage_of(S);
-      const uint4* r = static_cast<const uint4*>(f.cells) + ci * 4;
+      const uint4* r = static_cast<const uint4*>(f.cells) + uint64_t(ci) * 4;
       const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
-      a0 = dec16(r0.x, st); a1 = dec16(r0.x >> 16, st); a2 = dec16(r0.y, st);
-      b0 = dec16(r0.y >> 16, st); b1 = dec16(r0.z, st); b2 = dec16(r0.z >> 16, st);
-      c0 = dec16(r0.w, st); c1 = dec16(r0.w >> 16, st); c2 = dec16(r1.x, st);
-      e0 = dec16(r1.x >> 16, st); e1 = dec16(r1.y, st); e2 = dec16(r1.y >> 16, st);
-      g0 = dec16(r1.z, st); g1 = dec16(r1.z >> 16, st); g2 = dec16(r1.w, st);
-      h0 = dec16(r1.w >> 16, st); h1 = dec16(r2.x, st); h2 = dec16(r2.x >> 16, st);
-      k0 = dec16(r2.y, st); k1 = dec16(r2.y >> 16, st); k2 = dec16(r2.z, st);
-      l0 = dec16(r2.z >> 16, st); l1 = dec16(r2.w, st); l2 = dec16(r2.w >> 16, st);
+      a0 = dec16(r0.x, st); b0 = dec16(r0.x >> 16, st); c0 = dec16(r0.y, st); e0 = dec16(r0.y >> 16, st);
+      g0 = dec16(r0.z, st); h0 = dec16(r0.z >> 16, st); k0 = dec16(r0.w, st); l0 = dec16(r0.w >> 16, st);
+      a1 = dec16(r1.x, st); b1 = dec16(r1.x >> 16, st); c1 = dec16(r1.y, st); e1 = dec16(r1.y >> 16, st);
+      g1 = dec16(r1.z, st); h1 = dec16(r1.z >> 16, st); k1 = dec16(r1.w, st); l1 = dec16(r1.w >> 16, st);
+      a2 = dec16(r2.x, st); b2 = dec16(r2.x >> 16, st); c2 = dec16(r2.y, st); e2 = dec16(r2.y >> 16, st);
+      g2 = dec16(r2.z, st); h2 = dec16(r2.z >> 16, st); k2 = dec16(r2.w, st); l2 = dec16(r2.w >> 16, st);
     }
   } else {
     constexpr int SN = storage_of(S);
-    const int64_t i000 = static_cast<int64_t>(m[0]) * f.stride0 + static_cast<int64_t>(m[1]) * f.stride1 + m[2];
-    const int64_t i010 = i000 + f.stride1;
-    const int64_t i100 = i000 + f.stride0;
-    const int64_t i110 = i100 + f.stride1;
+    // fewer than 2^32 nodes (checked when the field is made)
+    const uint32_t s0 = uint32_t(f.stride0), s1 = uint32_t(f.stride1);
+    const uint32_t i000 = uint32_t(m[0]) * s0 + uint32_t(m[1]) * s1 + uint32_t(m[2]);
+    const uint32_t i010 = i000 + s1;
+    const uint32_t i100 = i000 + s0;
+    const uint32_t i110 = i100 + s1;
     load_node<SN>(f.data, i000, a0, a1, a2);
     load_node<SN>(f.data, i000 + 1, b0, b1, b2);
     load_node<SN>(f.data, i010, c0, c1, c2);
@@ -187,6 +224,8 @@ __device__ __forceinline__ void interpolate(const FieldDesc& f, const float d[3]
     load_node<SN>(f.data, i110, k0, k1, k2);
     load_node<SN>(f.data, i110 + 1, l0, l1, l2);
   }
+  float w[3], u[3];
+  cell_weights<INTERP>(f, d, q, m, w, u);
   // c00 = lerp(c000, c001, w2), c01 = lerp(c010, c011, w2), ...
   const float c00x = lerp1(a0, b0, w[2], u[2]), c00y = lerp1(a1, b1, w[2], u[2]), c00z = lerp1(a2, b2, w[2], u[2]);
   const float c01x = lerp1(c0, e0, w[2], u[2]), c01y = lerp1(c1, e1, w[2], u[2]), c01z = lerp1(c2, e2, w[2], u[2]);
@@ -201,9 +240,15 @@ __device__ __forceinline__ void interpolate(const FieldDesc& f, const float d[3]
 
 // Stop test (particle_tracer.cpp:312): Eigen isZero() with the float
 // dummy precision 1e-5f, or the |v| < eps extension.
+enum { STOP_RUNTIME = -1 };  // the rule is a runtime argument, selected without a branch
+
 template <int STOP>
-__device__ __forceinline__ bool stops(const float v[3], float eps2) {
-  if constexpr (STOP == PA_STOP_NORM) {
+__device__ __forceinline__ bool stops(const float v[3], float eps2, int rule = STOP) {
+  if constexpr (STOP == STOP_RUNTIME) {
+    const float n2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    const bool zero = (fabsf(v[0]) <= 1e-5f) & (fabsf(v[1]) <= 1e-5f) & (fabsf(v[2]) <= 1e-5f);
+    return rule == PA_STOP_NORM ? n2 < eps2 : zero;
+  } else if constexpr (STOP == PA_STOP_NORM) {
     const float n2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
     return n2 < eps2;
   } else {
@@ -274,14 +319,11 @@ __device__ __forceinline__ int face_rule(const FieldDesc& f, float4 p, int mask)
 
 __device__ __forceinline__ float4 termination_vertex() { return make_float4(-2.0f, -2.0f, -2.0f, 0.0f); }
 
-// One field evaluation plus one step from `last`.  Returns 0 = stepped
-// (next valid), 1 = left the block (contains failed), 2 = stop rule.
+// The stop test and the step of advance() once v = field(last) is known.
 template <int S, int MODE, int INTERP, int STOP>
-__device__ __forceinline__ int advance(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next) {
-  float d[3], q[3], v[3];
-  if (!locate(f, last.x, last.y, last.z, d, q)) return 1;
-  interpolate<S, INTERP>(f, d, q, v);
-  if (stops<STOP>(v, c.stop_eps2)) return 2;
+__device__ __forceinline__ int finish_step(const TraceConfig& c, const FieldDesc& f, float4 last, const float v[3],
+                                           float4& next, int rule = STOP) {
+  if (stops<STOP>(v, c.stop_eps2, rule)) return 2;
   if constexpr (MODE == STEP_CLASSIC) {
     // A stage point outside the block: take the frozen-derivative step so
     // the particle reaches the face and is handed off by the next contains.
@@ -290,6 +332,27 @@ __device__ __forceinline__ int advance(const TraceConfig& c, const FieldDesc& f,
     next = step_frozen<MODE>(c, last, v);
   }
   return 0;
+}
+
+// One field evaluation plus one step from `last`.  Returns 0 = stepped
+// (next valid), 1 = left the block (contains failed), 2 = stop rule.
+template <int S, int MODE, int INTERP, int STOP>
+__device__ __forceinline__ int advance(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next) {
+  float d[3], q[3], v[3];
+  if (!locate(f, last.x, last.y, last.z, d, q)) return 1;
+  interpolate<S, INTERP>(f, d, q, v);
+  return finish_step<S, MODE, INTERP, STOP>(c, f, last, v, next);
+}
+
+// advance() with the stop rule as a runtime argument, so a step loop that
+// serves both rules has a single body.
+template <int S, int MODE, int INTERP>
+__device__ __forceinline__ int advance_rt(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next,
+                                          int rule) {
+  float d[3], q[3], v[3];
+  if (!locate(f, last.x, last.y, last.z, d, q)) return 1;
+  interpolate<S, INTERP>(f, d, q, v);
+  return finish_step<S, MODE, INTERP, STOP_RUNTIME>(c, f, last, v, next, rule);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on

@@ -34,6 +34,8 @@ struct FieldDesc {
   int64_t cstride0;   // (dims[1]-1) * (dims[2]-1)
   int64_t cstride1;   // dims[2]-1
   int32_t layout;     // pa_layout
+  float inv_spacing[3];  // RN(1 / spacing), for the corrected quotient
+  int32_t fast_div;      // spacing within [2^-20, 2^20] on every axis
 };
 
 // Per-launch tracer configuration.  Slot 0 is the own block (vector field
@@ -130,6 +132,10 @@ struct CoopLaunch {
   int32_t* hand_code = nullptr;
 };
 pa_status launch_coop(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, bool union_form, const CoopLaunch& l);
+// Union trace on the quad engine into chunked slots; steps, ncont and work
+// are zeroed device counters.
+pa_status launch_quad_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
+                            unsigned long long* steps, unsigned* ncont, unsigned long long* work);
 
 // Exclusive scan of n int64 values in place on the device (n+1 outputs:
 // out[n] = total).  Uses ctx scratch.

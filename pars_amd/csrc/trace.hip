@@ -107,8 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TraceConfig cfg, const p
     for (; it < R; ++it) {
       if (it == R - 1) break;
       float4 next;
-      const int a = stop_rule == PA_STOP_NORM ? advance<S, MODE, INTERP, PA_STOP_NORM>(cfg, f, last, next)
-                                              : advance<S, MODE, INTERP, PA_STOP_ISZERO>(cfg, f, last, next);
+      const int a = advance_rt<S, MODE, INTERP>(cfg, f, last, next, stop_rule);
       if (a != 0) {
         exited = a == 1;
         break;
@@ -187,8 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, c
       for (; it < R; ++it) {
         if (it == R - 1) break;
         float4 next;
-        const int a = stop_rule == PA_STOP_NORM ? advance<S, MODE, INTERP, PA_STOP_NORM>(cfg, f, last, next)
-                                                : advance<S, MODE, INTERP, PA_STOP_ISZERO>(cfg, f, last, next);
+        const int a = advance_rt<S, MODE, INTERP>(cfg, f, last, next, stop_rule);
         if (a != 0) {
           exited = a == 1;
           break;
@@ -197,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, c
         last = next;
         ++steps;
       }
-      if (R < 2) {
+        if (R < 2) {
         used = base + 1;
         break;
       }
@@ -532,7 +530,8 @@ pa_status pa_gpu_tracer_set_stop_rule(pa_gpu_tracer* t, pa_stop_rule rule, float
 
 pa_status pa_gpu_tracer_set_engine(pa_gpu_tracer* t, pa_engine engine, int32_t waves) {
   if (!t) return PA_E_INVALID;
-  if (engine != PA_ENGINE_LANE && engine != PA_ENGINE_COOPERATIVE) return fail(t->ctx, PA_E_INVALID, "unknown engine");
+  if (engine != PA_ENGINE_LANE && engine != PA_ENGINE_COOPERATIVE && engine != PA_ENGINE_QUAD)
+    return fail(t->ctx, PA_E_INVALID, "unknown engine");
   if (waves < 0) return fail(t->ctx, PA_E_INVALID, "waves must be >= 0");
   t->engine = engine;
   t->waves = waves;
@@ -723,17 +722,19 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
   if (maxr < 0) return fail(ctx, PA_E_INVALID, "negative remaining_iterations");
   if (region_slots < size_t(maxr < 1 ? 1 : maxr))
     return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
-  if (order != PA_SLOTS_SEED_MAJOR && order != PA_SLOTS_STEP_MAJOR)
+  if (order != PA_SLOTS_SEED_MAJOR && order != PA_SLOTS_STEP_MAJOR && order != PA_SLOTS_CHUNKED)
     return fail(ctx, PA_E_INVALID, "unknown slot order");
   if (order == PA_SLOTS_STEP_MAJOR && t->engine != PA_ENGINE_LANE)
     return fail(ctx, PA_E_UNSUPPORTED, "step-major slots need the lane engine");
+  if ((order == PA_SLOTS_CHUNKED) != (t->engine == PA_ENGINE_QUAD))
+    return fail(ctx, PA_E_UNSUPPORTED, "the quad engine writes (only) chunked slots");
   const int64_t seed_stride = order == PA_SLOTS_STEP_MAJOR ? 1 : int64_t(region_slots);
   const int64_t slot_stride = order == PA_SLOTS_STEP_MAJOR ? int64_t(n) : 1;
   char* cbase = static_cast<char*>(ctx->counters);
   auto* steps = reinterpret_cast<unsigned long long*>(cbase + 1024);
   auto* ncont = reinterpret_cast<unsigned*>(cbase + 1032);
   const int mode = mode_of(t->integrator);
-  if (t->engine == PA_ENGINE_COOPERATIVE) {
+  if (t->engine == PA_ENGINE_COOPERATIVE || t->engine == PA_ENGINE_QUAD) {
     CoopLaunch L;
     L.particles = d_particles;
     L.n = int64_t(n);
@@ -745,7 +746,16 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
     L.cont = d_continuation;
     L.endpoints = reinterpret_cast<float4*>(d_endpoints);
     L.seed_steps = d_seed_steps;
-    PA_TRY(launch_coop(ctx, cfg, mode, true, L));
+    if (t->engine == PA_ENGINE_QUAD) {
+      auto* work = reinterpret_cast<unsigned long long*>(cbase + 1040);
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 3);
+      launch_begin(ctx, "trace_union");
+      const pa_status st = launch_quad_union(ctx, cfg, mode, L, steps, ncont, work);
+      launch_end(ctx);
+      if (st != PA_OK) return st;
+    } else {
+      PA_TRY(launch_coop(ctx, cfg, mode, true, L));
+    }
   } else {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 2);
   launch_begin(ctx, "trace_union");

@@ -58,9 +58,11 @@ template <int S>
 __device__ __forceinline__ void decode(const uint32_t* d, float* c) {
   using G = Gather<S>;
   if constexpr (G::ST == PA_STORAGE_FP32) {
-    if constexpr (G::CELLS) {
+    if constexpr (G::CELLS) {  // record: x of the 8 corners, then y, then z
 #pragma unroll
-      for (int i = 0; i < 24; ++i) c[i] = __uint_as_float(d[i]);
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) c[3 * k + j] = __uint_as_float(d[8 * j + k]);
     } else {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
@@ -72,10 +74,12 @@ __device__ __forceinline__ void decode(const uint32_t* d, float* c) {
   } else {
     if constexpr (G::CELLS) {
 #pragma unroll
-      for (int i = 0; i < 12; ++i) {
-        c[2 * i] = dec16(d[i], G::ST);
-        c[2 * i + 1] = dec16(d[i] >> 16, G::ST);
-      }
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int h = 8 * j + k;  // 16-bit element
+          c[3 * k + j] = dec16(d[h >> 1] >> (16 * (h & 1)), G::ST);
+        }
     } else {
       // piece zp = nodes (zp,0) and (zp,1): x|y<<16, z|pad, x|y<<16, z|pad
 #pragma unroll

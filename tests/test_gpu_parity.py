@@ -143,7 +143,7 @@ def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout, engine):
 @pytest.mark.parametrize("region_extra", [64, 1])
 @pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
 @pytest.mark.parametrize("engine,order", [(pa.ENGINE_COOPERATIVE, "seed"), (pa.ENGINE_LANE, "seed"),
-                                          (pa.ENGINE_LANE, "step")])
+                                          (pa.ENGINE_LANE, "step"), (pa.ENGINE_QUAD, "chunked")])
 @pytest.mark.parametrize("integ", [pa.RUNGE_KUTTA_4, pa.CLASSIC_RK4])
 def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, order, integ):
     """Union-of-7 continuation vs the oracle's round emulation: every seed's
@@ -157,6 +157,8 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, o
     ou = orc.trace_union(seeds, S.orc[0], S.orc[1:], sh.neighbor_ranks, sh.face_masks, dt=dt,
                          integrator=oracle_integrator(orc, integ))
     region = R + region_extra
+    if order == "chunked":  # multiple of 8; the tiny case leaves no room for a second segment
+        region = R if region_extra == 1 else R + 64
     n = len(seeds)
     pending = pa.Particles.from_host(ctx, seeds)
     seed_of = np.arange(n)
@@ -167,13 +169,21 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, o
         steps += r["total_steps"]
         cnt = r["counts"].download(np.int32, pending.n)
         v = r["vertices"].download(np.float32, pending.n * region * 4)
-        v = v.reshape(region, pending.n, 4).transpose(1, 0, 2) if order == "step" else v.reshape(pending.n, region, 4)
+        if order == "step":
+            v = v.reshape(region, pending.n, 4).transpose(1, 0, 2)
+        elif order == "chunked":  # [seed][chunk][component][8]
+            v = v.reshape(pending.n, region // 8, 4, 8).transpose(0, 1, 3, 2).reshape(pending.n, region, 4)
+        else:
+            v = v.reshape(pending.n, region, 4)
         for j in range(pending.n):
             got[int(seed_of[j])].append(v[j, :cnt[j]])
         total = int(cnt.sum())
         if order == "step":
             packed, m, offs = pa.particle_tracer.prune_counted_step_major(ctx, r["vertices"].ptr, r["counts"],
                                                                           pending.n, total)
+        elif order == "chunked":
+            packed, m, offs = pa.particle_tracer.prune_counted_chunked(ctx, r["vertices"].ptr, region, r["counts"],
+                                                                       pending.n, total)
         else:
             packed, m, offs = pa.particle_tracer.prune_counted(ctx, r["vertices"].ptr, region, r["counts"],
                                                                pending.n, total)
@@ -185,8 +195,8 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, o
         if cont.n == 0:
             break
         cp = cont.download()
-        order = np.argsort(cp["original_voxel"][:, 0], kind="stable")
-        cp = cp[order]
+        perm = np.argsort(cp["original_voxel"][:, 0], kind="stable")
+        cp = cp[perm]
         seed_of = seed_of[cp["original_voxel"][:, 0]]
         pending = pa.Particles.from_host(ctx, cp)
     else:
