@@ -47,7 +47,7 @@ def log(*a):
 class Pipeline:
     """Device buffers and calls of one step, reused across steps."""
 
-    def __init__(self, pa, ctx, fields, share, storage, interp, n_max, order="step"):
+    def __init__(self, pa, ctx, fields, share, storage, interp, n_max, order="step", engine=None):
         self.pa, self.ctx, self.lib = pa, ctx, pa.lib()
         self.order = order
         self.share = share
@@ -61,6 +61,8 @@ class Pipeline:
         t.set_interpolation(interp)
         if order == "chunked":
             t.set_engine(pa.ENGINE_QUAD)
+        elif engine is not None:
+            t.set_engine(engine)
         self.tracer = t
         self.region = R + EXTRA_SLOTS
         if order == "chunked":
@@ -109,8 +111,8 @@ class Pipeline:
         return dict(particles=n.value, steps=r["total_steps"], vertices=m.value, indices=ni.value)
 
 
-def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order):
-    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order)
+def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order, engine=None):
+    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order, engine)
     for _ in range(warmup):
         info = p.step()
     ctx.synchronize()
@@ -119,8 +121,12 @@ def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, or
     ctx.synchronize()
     t0 = time.perf_counter()
     total = 0
+    lat = []
     for _ in range(steps):
+        t1 = time.perf_counter()
         info = p.step()
+        ctx.synchronize()
+        lat.append(time.perf_counter() - t1)
         total += info["steps"]
     ctx.synchronize()
     wall = time.perf_counter() - t0
@@ -128,6 +134,8 @@ def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, or
     kt = {k: ctx.kernel_time(k) for k in ("seeds", "trace_union", "prune_scan", "prune_counted", "index", "color")}
     trace_ms = kt["trace_union"][0] / max(kt["trace_union"][1], 1)
     return p, dict(wall_s=wall, ms_per_step=1e3 * wall / steps, steps_per_pass=total // steps,
+                   pass_latency_ms=dict(p50=1e3 * float(np.percentile(lat, 50)),
+                                        p99=1e3 * float(np.percentile(lat, 99)), max=1e3 * max(lat)),
                    value=total / wall, trace_ms=trace_ms, info=info,
                    stage_ms={k: v[0] / max(v[1], 1) for k, v in kt.items()})
 
@@ -160,7 +168,7 @@ def endpoint_errors(pa, orc, p, share, storage, interp, sample_stride):
                 p99_endpoint_err=float(np.percentile(d, 99)) if len(d) else None)
 
 
-def cpu_baseline(orc, share, sample_stride, threads):
+def cpu_baseline(orc, share, sample_stride, threads, min_seconds=10.0):
     """The oracle's FP32 restatement of the same union trace on host cores."""
     from pars_amd import workloads as W
 
@@ -170,13 +178,44 @@ def cpu_baseline(orc, share, sample_stride, threads):
     seeds = orc.seed_generate(share.local.offset, share.local.size, share.local.spacing * np.float32(2), R, 21)
     sample = seeds[::sample_stride].copy()
     orc.set_threads(threads)
+    steps, passes = 0, 0
     t0 = time.perf_counter()
-    u = orc.trace_union(sample, f32[0], f32[1:], share.neighbor_ranks, share.face_masks, dt=DT, integrator=orc.RK4,
-                        stop_rule=orc.STOP_NORM, stop_eps=1e-6, vertex_capacity=len(sample) * (R + EXTRA_SLOTS))
+    while passes == 0 or time.perf_counter() - t0 < min_seconds:
+        u = orc.trace_union(sample, f32[0], f32[1:], share.neighbor_ranks, share.face_masks, dt=DT,
+                            integrator=orc.RK4, stop_rule=orc.STOP_NORM, stop_eps=1e-6,
+                            vertex_capacity=len(sample) * (R + EXTRA_SLOTS))
+        steps += u["total_steps"]
+        passes += 1
     dt = time.perf_counter() - t0
-    return dict(value=u["total_steps"] / dt, unit="particle-steps/s", cores=threads, kind="port",
-                sample="every %dth seed of the 262144 (%d seeds, %d steps), union trace with vertex output, "
-                       "%.2f s" % (sample_stride, len(sample), u["total_steps"], dt))
+    return dict(value=steps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample="%d passes of the union trace with vertex output over %s of the 262144 seeds "
+                       "(%d seeds, %d steps per pass), %.1f s" %
+                       (passes, "all" if sample_stride == 1 else "every %dth" % sample_stride, len(sample),
+                        u["total_steps"], dt))
+
+
+def kernel_name(form):
+    tokens = form.split("-")
+    st = tokens[0].upper()
+    lay = "cells" if "cells" in tokens else "nodes"
+    interp = "consistent" if "consistent" in tokens else "reference"
+    if "lane" in tokens:
+        k = "k_trace_union"
+    elif "quad" in tokens:
+        k = "k_quad_union"
+    else:
+        k = "k_hybrid_union"
+    return "%s<%s %s, RK4, %s>" % (k, st, lay, interp)
+
+
+def bf16_ratio(forms, extra):
+    """BF16 over FP32 with the headline form's layout and engine."""
+    head = forms[0]
+    twin = "bf16" + head[len("fp32"):] if head.startswith("fp32") else None
+    if twin not in extra:
+        return None
+    return dict(form=twin, pass_ratio=extra[twin]["value"] / extra[head]["value"],
+                trace_ratio=extra[twin]["trace_steps_per_s"] / extra[head]["trace_steps_per_s"])
 
 
 def load_traffic(name):
@@ -192,7 +231,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--forms", default="fp32,fp32-cells,bf16-cells,fp16-cells,fp32-cells-consistent")
+    ap.add_argument("--forms", default="fp32-cells,fp32,bf16-cells,bf16,fp16-cells,fp32-cells-consistent,"
+                                       "fp32-cells-lane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
@@ -238,7 +278,8 @@ def main():
         interp = pa.INTERP_CONSISTENT if "consistent" in form.split("-") else pa.INTERP_REFERENCE
         tokens = form.split("-")
         order = "seed" if "seedmajor" in tokens else "chunked" if "quad" in tokens else "step"
-        p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order)
+        engine = pa.ENGINE_LANE if "lane" in tokens else pa.ENGINE_HYBRID if order == "step" else None
+        p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order, engine)
         log("%s: %.3f ms/step, %.3e steps/s, trace kernel %.3f ms, %s" %
             (form, res["ms_per_step"], res["value"], res["trace_ms"], res["info"]))
         results[form] = res
@@ -257,10 +298,10 @@ def main():
     st0 = storage_of[forms[0]]
     alg_bytes = steps_per_pass * (GATHER_BYTES[st0] + STORE_BYTES)
     achieved = alg_bytes / (base["trace_ms"] * 1e-3) / 1e9
-    traffic = load_traffic("trace_union_config2_fp32.traffic.json")
+    traffic = load_traffic("trace_union_share_fp32_cells.traffic.json")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                     traffic=(traffic or {}).get("bytes_per_launch"),
-                    kernel="trace_union (k_trace_union<FP32, RK4, reference>)",
+                    kernel="trace_union, form %s (%s)" % (forms[0], kernel_name(forms[0])),
                     algorithmic_bytes_per_step=GATHER_BYTES[st0] + STORE_BYTES,
                     steps_per_launch=steps_per_pass, avg_launch_ms=base["trace_ms"],
                     effective_gather_gbs=steps_per_pass * GATHER_BYTES[st0] / (base["trace_ms"] * 1e-3) / 1e9)
@@ -270,6 +311,7 @@ def main():
         r = results[form]
         st = storage_of[form]
         extra[form] = dict(value=r["value"], ms_per_step=r["ms_per_step"], trace_kernel_ms=r["trace_ms"],
+                           kernel=kernel_name(form), pass_latency_ms=r["pass_latency_ms"],
                            trace_steps_per_s=r["steps_per_pass"] / (r["trace_ms"] * 1e-3),
                            steps_per_pass=r["steps_per_pass"], stage_ms=r["stage_ms"],
                            vs_fp32=r["value"] / base["value"],
@@ -291,7 +333,7 @@ def main():
             log("errors %s (%.1f s): %s" % (form, time.perf_counter() - t0, extra[form]["fp64_error"]))
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(orc, share, 2, args.cpu_threads)
+        cpu = cpu_baseline(orc, share, 1, args.cpu_threads)
         log("cpu baseline:", cpu)
 
     line = dict(
@@ -299,14 +341,16 @@ def main():
         warmup=args.warmup, ms_per_step=base["ms_per_step"], higher_is_better=True, scaling="weak",
         vs_baseline=None, dtype="f32", data="synthetic",
         config=dict(workload="one rank's share: ABC 512^3 split 4x4x4, rank 21 + 6 face neighbours "
-                             "(7 x 129^3 ghosted FP32 vec4 blocks, 240 MB), 262144 seeds stride 2, "
+                             "(7 x 129^3 ghosted FP32 vec4 blocks, 240 MB of nodes; cell-record layout "
+                             "2.1 GB), 262144 seeds stride 2, "
                              "runge_kutta_4 (reference frozen-derivative stepper), dt 0.005, <= 1000 steps, "
                              "union-of-7 continuation, stop |u| < 1e-6; step = seeds + trace + prune + index + "
                              "colour (hsv_constant_s 0.75)",
                     seeds=n_seeds, remaining_iterations=R, dt=DT, field="ABC A=sqrt3 B=sqrt2 C=1",
-                    interpolation="reference" if forms[0] == "fp32" else forms[0], device=dev["name"]),
+                    form=forms[0], device=dev["name"]),
+        pass_latency_ms=base["pass_latency_ms"],
         roofline=roofline, cpu_baseline=cpu, forms=extra,
-        bf16_vs_fp32=extra.get("bf16", {}).get("vs_fp32"),
+        bf16_vs_fp32=bf16_ratio(forms, extra),
     )
     print(json.dumps(line), flush=True)
 

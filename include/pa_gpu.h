@@ -241,10 +241,18 @@ pa_status   pa_gpu_tracer_set_interpolation(pa_gpu_tracer* tracer, pa_interp int
  * counter.  PA_ENGINE_QUAD (union trace only, chunked slots) splits every
  * particle over the four lanes of a quad, one vec4 component each, which
  * shortens the serial step chain that the longest trajectories of a round
- * sit on; persistent waves refill quads from a work counter.  All engines
- * produce identical results; PA_ENGINE_LANE is the default.  waves =
- * resident waves of the cooperative engine (0 = automatic). */
-typedef enum pa_engine { PA_ENGINE_LANE = 0, PA_ENGINE_COOPERATIVE = 1, PA_ENGINE_QUAD = 2 } pa_engine;
+ * sit on; persistent waves refill quads from a work counter.
+ * PA_ENGINE_HYBRID (union trace only, step-major slots) runs lane mode while
+ * a wave holds more than 16 live particles and moves the survivors onto
+ * quads after that.  All engines produce identical results;
+ * PA_ENGINE_LANE is the default.  waves = resident waves of the cooperative
+ * engine (0 = automatic). */
+typedef enum pa_engine {
+  PA_ENGINE_LANE = 0,
+  PA_ENGINE_COOPERATIVE = 1,
+  PA_ENGINE_QUAD = 2,
+  PA_ENGINE_HYBRID = 3
+} pa_engine;
 pa_status   pa_gpu_tracer_set_engine(pa_gpu_tracer* tracer, pa_engine engine, int32_t waves);
 /* Extension: XCD-aware workgroup order of the lane engine (default on):
  * the 8 XCDs, which take workgroups round-robin, each process a contiguous

@@ -51,7 +51,7 @@ INTEGRATORS = {
 STOP_ISZERO, STOP_NORM = 0, 1
 INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
 LAYOUT_NODES, LAYOUT_CELLS = 0, 1
-ENGINE_LANE, ENGINE_COOPERATIVE, ENGINE_QUAD = 0, 1, 2
+ENGINE_LANE, ENGINE_COOPERATIVE, ENGINE_QUAD, ENGINE_HYBRID = 0, 1, 2, 3
 SLOTS_SEED_MAJOR, SLOTS_STEP_MAJOR, SLOTS_CHUNKED = 0, 1, 2
 SLOT_ORDERS = {"seed": SLOTS_SEED_MAJOR, "step": SLOTS_STEP_MAJOR, "chunked": SLOTS_CHUNKED}
 TERMINATION_VERTEX = np.array([-2.0, -2.0, -2.0, 0.0], np.float32)
@@ -386,8 +386,9 @@ class particle_tracer:
         self.ctx.check(lib().pa_gpu_tracer_set_step_size(self._h, C.c_float(step_size)))
 
     def set_engine(self, engine: int, waves: int = 0):
-        """ENGINE_LANE (default), ENGINE_COOPERATIVE or ENGINE_QUAD (union trace, chunked slots);
-        waves = resident waves of the cooperative engine (0 = automatic)."""
+        """ENGINE_LANE (default), ENGINE_COOPERATIVE, ENGINE_QUAD (union trace, chunked slots) or
+        ENGINE_HYBRID (union trace, step-major slots); waves = resident waves of the cooperative
+        engine (0 = automatic)."""
         self.ctx.check(lib().pa_gpu_tracer_set_engine(self._h, C.c_int(engine), C.c_int32(waves)), "set_engine")
 
     def set_xcd_remap(self, enable: bool):

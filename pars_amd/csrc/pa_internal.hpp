@@ -136,6 +136,10 @@ pa_status launch_coop(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, bool un
 // are zeroed device counters.
 pa_status launch_quad_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
                             unsigned long long* steps, unsigned* ncont, unsigned long long* work);
+// Union trace on the hybrid engine (lane mode, then quad mode) into
+// step-major slots.
+pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
+                              unsigned long long* steps, unsigned* ncont);
 
 // Exclusive scan of n int64 values in place on the device (n+1 outputs:
 // out[n] = total).  Uses ctx scratch.

@@ -530,7 +530,8 @@ pa_status pa_gpu_tracer_set_stop_rule(pa_gpu_tracer* t, pa_stop_rule rule, float
 
 pa_status pa_gpu_tracer_set_engine(pa_gpu_tracer* t, pa_engine engine, int32_t waves) {
   if (!t) return PA_E_INVALID;
-  if (engine != PA_ENGINE_LANE && engine != PA_ENGINE_COOPERATIVE && engine != PA_ENGINE_QUAD)
+  if (engine != PA_ENGINE_LANE && engine != PA_ENGINE_COOPERATIVE && engine != PA_ENGINE_QUAD &&
+      engine != PA_ENGINE_HYBRID)
     return fail(t->ctx, PA_E_INVALID, "unknown engine");
   if (waves < 0) return fail(t->ctx, PA_E_INVALID, "waves must be >= 0");
   t->engine = engine;
@@ -724,8 +725,10 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
     return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
   if (order != PA_SLOTS_SEED_MAJOR && order != PA_SLOTS_STEP_MAJOR && order != PA_SLOTS_CHUNKED)
     return fail(ctx, PA_E_INVALID, "unknown slot order");
-  if (order == PA_SLOTS_STEP_MAJOR && t->engine != PA_ENGINE_LANE)
-    return fail(ctx, PA_E_UNSUPPORTED, "step-major slots need the lane engine");
+  if (order == PA_SLOTS_STEP_MAJOR && t->engine != PA_ENGINE_LANE && t->engine != PA_ENGINE_HYBRID)
+    return fail(ctx, PA_E_UNSUPPORTED, "step-major slots need the lane or hybrid engine");
+  if (t->engine == PA_ENGINE_HYBRID && order != PA_SLOTS_STEP_MAJOR)
+    return fail(ctx, PA_E_UNSUPPORTED, "the hybrid engine writes step-major slots");
   if ((order == PA_SLOTS_CHUNKED) != (t->engine == PA_ENGINE_QUAD))
     return fail(ctx, PA_E_UNSUPPORTED, "the quad engine writes (only) chunked slots");
   const int64_t seed_stride = order == PA_SLOTS_STEP_MAJOR ? 1 : int64_t(region_slots);
@@ -734,7 +737,7 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
   auto* steps = reinterpret_cast<unsigned long long*>(cbase + 1024);
   auto* ncont = reinterpret_cast<unsigned*>(cbase + 1032);
   const int mode = mode_of(t->integrator);
-  if (t->engine == PA_ENGINE_COOPERATIVE || t->engine == PA_ENGINE_QUAD) {
+  if (t->engine == PA_ENGINE_COOPERATIVE || t->engine == PA_ENGINE_QUAD || t->engine == PA_ENGINE_HYBRID) {
     CoopLaunch L;
     L.particles = d_particles;
     L.n = int64_t(n);
@@ -746,7 +749,13 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
     L.cont = d_continuation;
     L.endpoints = reinterpret_cast<float4*>(d_endpoints);
     L.seed_steps = d_seed_steps;
-    if (t->engine == PA_ENGINE_QUAD) {
+    if (t->engine == PA_ENGINE_HYBRID) {
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 2);
+      launch_begin(ctx, "trace_union");
+      const pa_status st = launch_hybrid_union(ctx, cfg, mode, L, steps, ncont);
+      launch_end(ctx);
+      if (st != PA_OK) return st;
+    } else if (t->engine == PA_ENGINE_QUAD) {
       auto* work = reinterpret_cast<unsigned long long*>(cbase + 1040);
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 3);
       launch_begin(ctx, "trace_union");

@@ -201,20 +201,9 @@ __device__ __forceinline__ int quad_face(const QLane& L, int c, float x) {
   return bits ? __builtin_ctz(static_cast<unsigned>(bits)) : -1;
 }
 
-enum { Q_IDLE = 0, Q_START = 1, Q_STEP = 2, Q_DONE = 3 };
-
-template <int S, int MODE, int INTERP>
-__global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
-                                                           int64_t n, float* __restrict__ slots, int64_t region,
-                                                           int32_t* __restrict__ counts,
-                                                           pa_particle* __restrict__ cont,
-                                                           unsigned int* __restrict__ n_cont,
-                                                           unsigned long long* __restrict__ steps_out,
-                                                           unsigned long long* __restrict__ work, int rule,
-                                                           float* __restrict__ endpoints,
-                                                           int32_t* __restrict__ seed_steps) {
-  __shared__ QEntry tab[7][4];
-  __shared__ QBlock blk[7];
+// Per-component block table in LDS (threads 0..27 of the workgroup).
+template <int S>
+__device__ __forceinline__ void fill_tables(const TraceConfig& cfg, QEntry (*tab)[4], QBlock* blk) {
   if (threadIdx.x < 28) {
     const int b = threadIdx.x >> 2, c = threadIdx.x & 3;
     const FieldDesc& F = cfg.field[b];
@@ -248,14 +237,118 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
       blk[b] = k;
     }
   }
+}
+
+enum { Q_IDLE = 0, Q_START = 1, Q_STEP = 2, Q_DONE = 3 };
+
+// One particle's union-trace state (replicated over the quad in quad mode).
+struct QState {
+  int state;
+  int64_t pid;
+  int b, R, it, slot, nsteps, orank;
+};
+
+// A continuation record for a seed whose region is full (as the lane engine).
+__device__ __forceinline__ void write_continuation(pa_particle* dst, const float4& last, int Rn, int nb, int orank,
+                                                   int64_t pid) {
+  dst->position[0] = last.x;
+  dst->position[1] = last.y;
+  dst->position[2] = last.z;
+  dst->position[3] = last.w;
+  dst->remaining_iterations = Rn;
+  dst->vector_field_index = nb;
+  dst->original_rank = orank;
+  dst->original_voxel[0] = int32_t(pid);
+  dst->original_voxel[1] = 0;
+  dst->original_voxel[2] = 0;
+  dst->reserved_[0] = 0;
+  dst->reserved_[1] = 0;
+}
+
+// One iteration of an active quad (Q_START or Q_STEP): produces exactly one
+// slot, returned as this lane's component; the slot index is st.slot - 1
+// afterwards.  Segment logic of the lane engine's trace_union.
+template <int S, int MODE, int INTERP>
+__device__ __forceinline__ float quad_iterate(const TraceConfig& cfg, const QEntry (*tab)[4], const QBlock* blk,
+                                              QState& st, QLane& L, float& last, int c, int rule, int64_t region,
+                                              pa_particle* cont, unsigned* n_cont) {
+  const float term = c == 3 ? 0.0f : -2.0f;  // termination vertex (-2, -2, -2, 0)
+  float val = term;
+  ++st.slot;
+  if (st.state == Q_START) {
+    val = last;
+    if (st.R < 2) {
+      st.state = Q_DONE;
+    } else {
+      st.state = Q_STEP;
+      st.it = 1;
+    }
+  } else if (st.it == st.R - 1) {
+    st.state = Q_DONE;
+  } else {
+    float next;
+    const int a = quad_advance<S, MODE, INTERP>(cfg, L, c, last, next, rule);
+    if (a == 0) {
+      val = next;
+      last = next;
+      ++st.it;
+      ++st.nsteps;
+    } else {
+      st.state = Q_DONE;
+      if (a == 1) {
+        const int face = quad_face(L, c, last);
+        int nb = -1;
+        bool go;
+        if (st.b < 0) {
+          go = face >= 0 && blk[face + 1].has != 0;
+          nb = face;
+        } else {
+          go = face == (st.b ^ 1);
+        }
+        if (go) {
+          const int Rn = st.R - st.it;
+          if (int64_t(st.slot) + Rn > region) {
+            unsigned idx = 0;
+            if (c == 0) idx = atomicAdd(n_cont, 1u);
+            idx = static_cast<unsigned>(qmov<kBcast0>(static_cast<int>(idx)));
+            const float lx = qmovf<kBcast0>(last), ly = qmovf<kBcast1>(last), lz = qmovf<kBcast2>(last),
+                        lw = qmovf<0xFF>(last);
+            if (c == 0) write_continuation(cont + idx, make_float4(lx, ly, lz, lw), Rn, nb, st.orank, st.pid);
+          } else {
+            st.b = nb;
+            st.R = Rn;
+            load_lane(tab, blk, st.b, c, L);
+            st.state = Q_START;
+          }
+        }
+      }
+    }
+  }
+  return val;
+}
+
+template <int S, int MODE, int INTERP>
+__global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
+                                                           int64_t n, float* __restrict__ slots, int64_t region,
+                                                           int32_t* __restrict__ counts,
+                                                           pa_particle* __restrict__ cont,
+                                                           unsigned int* __restrict__ n_cont,
+                                                           unsigned long long* __restrict__ steps_out,
+                                                           unsigned long long* __restrict__ work, int rule,
+                                                           float* __restrict__ endpoints,
+                                                           int32_t* __restrict__ seed_steps) {
+  __shared__ QEntry tab[7][4];
+  __shared__ QBlock blk[7];
+  fill_tables<S>(cfg, tab, blk);
   __syncthreads();
 
   const int lane = threadIdx.x & 63;
   const int c = lane & 3;
-  const float term = c == 3 ? 0.0f : -2.0f;  // termination vertex (-2, -2, -2, 0)
-  int state = Q_IDLE;
-  int64_t pid = 0;
-  int b = -1, R = 0, it = 0, slot = 0, nsteps = 0, orank = 0;
+  QState st;
+  st.state = Q_IDLE;
+  st.pid = 0;
+  st.b = -1;
+  st.R = st.it = st.slot = st.nsteps = st.orank = 0;
   float last = 0.0f;
   QLane L;
   load_lane(tab, blk, -1, c, L);
@@ -264,7 +357,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
 
   for (;;) {
     // Idle quads take the next seeds (all lanes are converged here).
-    const uint64_t leaders = __ballot(state == Q_IDLE && c == 0);
+    const uint64_t leaders = __ballot(st.state == Q_IDLE && c == 0);
     if (open && leaders) {
       const uint32_t take = uint32_t(__popcll(leaders));
       unsigned long long base = 0;
@@ -273,109 +366,216 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
       const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(base >> 32));
       base = (static_cast<unsigned long long>(hi) << 32) | lo;
       if (base + take >= static_cast<unsigned long long>(n)) open = false;
-      if (state == Q_IDLE) {
+      if (st.state == Q_IDLE) {
         const uint64_t below = (uint64_t(1) << (lane & ~3)) - 1;
         const unsigned long long p = base + uint32_t(__popcll(leaders & below));
         if (p < static_cast<unsigned long long>(n)) {
-          pid = int64_t(p);
-          const pa_particle& src = P[pid];
+          st.pid = int64_t(p);
+          const pa_particle& src = P[st.pid];
           last = src.position[c];
-          R = src.remaining_iterations;
-          b = src.vector_field_index;
-          orank = src.original_rank;
-          load_lane(tab, blk, b, c, L);
-          state = Q_START;
-          slot = 0;
-          nsteps = 0;
+          st.R = src.remaining_iterations;
+          st.b = src.vector_field_index;
+          st.orank = src.original_rank;
+          load_lane(tab, blk, st.b, c, L);
+          st.state = Q_START;
+          st.slot = 0;
+          st.nsteps = 0;
         }
       }
     }
-    if (__ballot(state != Q_IDLE) == 0) break;
+    if (__ballot(st.state != Q_IDLE) == 0) break;
 
-    const bool writes = state != Q_IDLE;
-    const int slot0 = slot;  // a multiple of kChunk for every writing quad
+    const bool writes = st.state != Q_IDLE;
+    const int slot0 = st.slot;  // a multiple of kChunk for every writing quad
     float buf[kChunk];
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) {
-      float val = term;
-      if (state == Q_START) {
+      float val = 0.0f;
+      if (st.state == Q_START || st.state == Q_STEP)
+        val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, st, L, last, c, rule, region, cont, n_cont);
+      buf[j] = val;
+    }
+    if (writes) {
+      float4* dst = reinterpret_cast<float4*>(slots + (st.pid * region + slot0) * 4 + c * kChunk);
+      dst[0] = make_float4(buf[0], buf[1], buf[2], buf[3]);
+      dst[1] = make_float4(buf[4], buf[5], buf[6], buf[7]);
+    }
+    if (st.state == Q_DONE) {
+      if (c == 0) {
+        counts[st.pid] = st.slot;
+        if (seed_steps) seed_steps[st.pid] = st.nsteps;
+        wave_steps += static_cast<unsigned long long>(st.nsteps);
+      }
+      if (endpoints) endpoints[st.pid * 4 + c] = last;
+      st.state = Q_IDLE;
+    }
+  }
+  wave_steps = wave_sum_u64(wave_steps);
+  if (lane == 0 && wave_steps) atomicAdd(steps_out, wave_steps);
+}
+
+// Hybrid union trace, step-major slots (slot s of seed p = vec4 at s * n + p,
+// as the lane engine's PA_SLOTS_STEP_MAJOR).  Every lane starts with one seed
+// (lane mode, all four components per lane: the most particles in flight
+// while the round is throughput-bound).  Once a wave has at most 16 live
+// particles, which is where the longest trajectories leave it latency-bound,
+// the survivors move through LDS onto quads (quad mode, one component per
+// lane) and finish there.  Both modes run the same per-iteration segment
+// machine, so the slots are those of the lane engine, bit for bit.
+constexpr int kQuadSwitch = 16;
+
+template <int S, int MODE, int INTERP>
+__global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
+                                                             int64_t n, float4* __restrict__ verts, int64_t region,
+                                                             int32_t* __restrict__ counts,
+                                                             pa_particle* __restrict__ cont,
+                                                             unsigned int* __restrict__ n_cont,
+                                                             unsigned long long* __restrict__ steps_out, int rule,
+                                                             float4* __restrict__ endpoints,
+                                                             int32_t* __restrict__ seed_steps) {
+  __shared__ QEntry tab[7][4];
+  __shared__ QBlock blk[7];
+  __shared__ float4 x_last[kQuadBlock / 64][kQuadSwitch];
+  __shared__ int32_t x_int[kQuadBlock / 64][kQuadSwitch][8];
+  fill_tables<S>(cfg, tab, blk);
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  unsigned long long wave_steps = 0;
+  QState st;
+  st.pid = logical_block(cfg.xcd_remap) * kQuadBlock + threadIdx.x;
+  st.state = st.pid < n ? Q_START : Q_IDLE;
+  st.b = -1;
+  st.R = st.it = st.slot = st.nsteps = st.orank = 0;
+  float4 last = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  FieldDesc f = cfg.field[0];
+  if (st.state == Q_START) {
+    const pa_particle p = P[st.pid];
+    last = make_float4(p.position[0], p.position[1], p.position[2], p.position[3]);
+    st.R = p.remaining_iterations;
+    st.b = p.vector_field_index;
+    st.orank = p.original_rank;
+    f = cfg.field[st.b + 1];
+  }
+
+  // ---- lane mode
+  uint64_t live;
+  for (;;) {
+    live = __ballot(st.state != Q_IDLE);
+    if (__popcll(live) <= kQuadSwitch) break;
+    if (st.state != Q_IDLE) {
+      float4 val = termination_vertex();
+      ++st.slot;
+      if (st.state == Q_START) {
         val = last;
-        ++slot;
-        if (R < 2) {
-          state = Q_DONE;
+        if (st.R < 2) {
+          st.state = Q_DONE;
         } else {
-          state = Q_STEP;
-          it = 1;
+          st.state = Q_STEP;
+          st.it = 1;
         }
-      } else if (state == Q_STEP) {
-        ++slot;
-        if (it == R - 1) {
-          state = Q_DONE;
+      } else if (st.it == st.R - 1) {
+        st.state = Q_DONE;
+      } else {
+        float4 next;
+        const int a = advance_rt<S, MODE, INTERP>(cfg, f, last, next, rule);
+        if (a == 0) {
+          val = next;
+          last = next;
+          ++st.it;
+          ++st.nsteps;
         } else {
-          float next;
-          const int a = quad_advance<S, MODE, INTERP>(cfg, L, c, last, next, rule);
-          if (a == 0) {
-            val = next;
-            last = next;
-            ++it;
-            ++nsteps;
-          } else if (a == 2) {
-            state = Q_DONE;
-          } else {
-            // hand-off within the union (trace_union of the lane engine)
-            state = Q_DONE;
-            const int face = quad_face(L, c, last);
+          st.state = Q_DONE;
+          if (a == 1) {
             int nb = -1;
             bool go;
-            if (b < 0) {
-              go = face >= 0 && blk[face + 1].has != 0;
-              nb = face;
+            if (st.b < 0) {
+              const int k = face_rule(f, last, cfg.local_mask);
+              go = k >= 0 && cfg.has_field[k + 1];
+              nb = k;
             } else {
-              go = face == (b ^ 1);
+              go = face_rule(f, last, cfg.face_mask[st.b]) == (st.b ^ 1);
             }
             if (go) {
-              const int Rn = R - it;
-              if (int64_t(slot) + Rn > region) {
-                unsigned idx = 0;
-                if (c == 0) idx = atomicAdd(n_cont, 1u);
-                idx = static_cast<unsigned>(qmov<kBcast0>(static_cast<int>(idx)));
-                pa_particle* dst = cont + idx;
-                dst->position[c] = last;
-                if (c == 0) {
-                  dst->remaining_iterations = Rn;
-                  dst->vector_field_index = nb;
-                  dst->original_rank = orank;
-                  dst->original_voxel[0] = int32_t(pid);
-                  dst->original_voxel[1] = 0;
-                  dst->original_voxel[2] = 0;
-                  dst->reserved_[0] = 0;
-                  dst->reserved_[1] = 0;
-                }
+              const int Rn = st.R - st.it;
+              if (int64_t(st.slot) + Rn > region) {
+                write_continuation(cont + atomicAdd(n_cont, 1u), last, Rn, nb, st.orank, st.pid);
               } else {
-                b = nb;
-                R = Rn;
-                load_lane(tab, blk, b, c, L);
-                state = Q_START;
+                st.b = nb;
+                st.R = Rn;
+                f = cfg.field[st.b + 1];
+                st.state = Q_START;
               }
             }
           }
         }
       }
-      buf[j] = val;
-    }
-    if (writes) {
-      float4* dst = reinterpret_cast<float4*>(slots + (pid * region + slot0) * 4 + c * kChunk);
-      dst[0] = make_float4(buf[0], buf[1], buf[2], buf[3]);
-      dst[1] = make_float4(buf[4], buf[5], buf[6], buf[7]);
-    }
-    if (state == Q_DONE) {
-      if (c == 0) {
-        counts[pid] = slot;
-        if (seed_steps) seed_steps[pid] = nsteps;
-        wave_steps += static_cast<unsigned long long>(nsteps);
+      verts[int64_t(st.slot - 1) * n + st.pid] = val;
+      if (st.state == Q_DONE) {
+        counts[st.pid] = st.slot;
+        if (seed_steps) seed_steps[st.pid] = st.nsteps;
+        if (endpoints) endpoints[st.pid] = last;
+        wave_steps += static_cast<unsigned long long>(st.nsteps);
+        st.state = Q_IDLE;
       }
-      if (endpoints) endpoints[pid * 4 + c] = last;
-      state = Q_IDLE;
+    }
+  }
+
+  // ---- quad mode: live particle k of the wave goes to quad k
+  const int nlive = __popcll(live);
+  if (nlive > 0) {
+    if (st.state != Q_IDLE) {
+      const int k = __popcll(live & ((uint64_t(1) << lane) - 1));
+      x_last[w][k] = last;
+      int32_t* xi = x_int[w][k];
+      xi[0] = st.state;
+      xi[1] = int32_t(st.pid);
+      xi[2] = st.b;
+      xi[3] = st.R;
+      xi[4] = st.it;
+      xi[5] = st.slot;
+      xi[6] = st.nsteps;
+      xi[7] = st.orank;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int q = lane >> 2, c = lane & 3;
+    QState qs;
+    float ql = 0.0f;
+    QLane L;
+    qs.state = Q_IDLE;
+    if (q < nlive) {
+      const float4 xl = x_last[w][q];
+      ql = c == 0 ? xl.x : c == 1 ? xl.y : c == 2 ? xl.z : xl.w;
+      const int32_t* xi = x_int[w][q];
+      qs.state = xi[0];
+      qs.pid = xi[1];
+      qs.b = xi[2];
+      qs.R = xi[3];
+      qs.it = xi[4];
+      qs.slot = xi[5];
+      qs.nsteps = xi[6];
+      qs.orank = xi[7];
+      load_lane(tab, blk, qs.b, c, L);
+    }
+    float* vf = reinterpret_cast<float*>(verts);
+    for (;;) {
+      if (__ballot(qs.state != Q_IDLE) == 0) break;
+      if (qs.state != Q_IDLE) {
+        const float val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont);
+        vf[(int64_t(qs.slot - 1) * n + qs.pid) * 4 + c] = val;
+        if (qs.state == Q_DONE) {
+          if (c == 0) {
+            counts[qs.pid] = qs.slot;
+            if (seed_steps) seed_steps[qs.pid] = qs.nsteps;
+            wave_steps += static_cast<unsigned long long>(qs.nsteps);
+          }
+          if (endpoints) reinterpret_cast<float*>(endpoints)[qs.pid * 4 + c] = ql;
+          qs.state = Q_IDLE;
+        }
+      }
     }
   }
   wave_steps = wave_sum_u64(wave_steps);
@@ -383,6 +583,19 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
 }
 
 }  // namespace
+
+pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
+                              unsigned long long* steps, unsigned* ncont) {
+  if (l.n > INT32_MAX) return fail(ctx, PA_E_INVALID, "hybrid engine: at most 2^31-1 seeds per call");
+  const int64_t blocks = (l.n + kQuadBlock - 1) / kQuadBlock;
+  dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
+    hipLaunchKernelGGL((k_hybrid_union<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
+                       dim3(unsigned(blocks)), dim3(kQuadBlock), 0, ctx->stream, cfg, l.particles, l.n, l.verts,
+                       l.region, l.counts, l.cont, ncont, steps, l.stop_rule, l.endpoints, l.seed_steps);
+  }, cfg.field[0].layout);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
 
 pa_status launch_quad_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
                             unsigned long long* steps, unsigned* ncont, unsigned long long* work) {

@@ -143,7 +143,8 @@ def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout, engine):
 @pytest.mark.parametrize("region_extra", [64, 1])
 @pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
 @pytest.mark.parametrize("engine,order", [(pa.ENGINE_COOPERATIVE, "seed"), (pa.ENGINE_LANE, "seed"),
-                                          (pa.ENGINE_LANE, "step"), (pa.ENGINE_QUAD, "chunked")])
+                                          (pa.ENGINE_LANE, "step"), (pa.ENGINE_QUAD, "chunked"),
+                                          (pa.ENGINE_HYBRID, "step")])
 @pytest.mark.parametrize("integ", [pa.RUNGE_KUTTA_4, pa.CLASSIC_RK4])
 def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, order, integ):
     """Union-of-7 continuation vs the oracle's round emulation: every seed's
