@@ -47,9 +47,10 @@ def log(*a):
 class Pipeline:
     """Device buffers and calls of one step, reused across steps."""
 
-    def __init__(self, pa, ctx, fields, share, storage, interp, n_max, order="step", engine=None):
+    def __init__(self, pa, ctx, fields, share, storage, interp, n_max, order="step", engine=None, fused=False):
         self.pa, self.ctx, self.lib = pa, ctx, pa.lib()
         self.order = order
+        self.fused = fused
         self.share = share
         t = pa.particle_tracer(ctx)
         t.set_local_vector_field(fields[0])
@@ -85,11 +86,28 @@ class Pipeline:
                                            C.c_void_p(self.seeds.ptr), C.c_size_t(self.seeds.nbytes // 48),
                                            C.byref(n)), "seeds")
         parts = pa.Particles(self.seeds, n.value)
+        if self.fused:
+            ncont, nv, ni, steps = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint64()
+            ctx.check(lib.pa_gpu_tracer_trace_union_round(
+                self.tracer._h, C.c_void_p(self.seeds.ptr), n, C.c_size_t(self.region),
+                C.c_void_p(self.ub["vertices"].ptr), C.c_void_p(self.ub["counts"].ptr),
+                C.c_void_p(self.ub["cont"].ptr), C.byref(ncont), C.c_int(2), C.c_float(0.75),
+                C.c_void_p(self.pruned.ptr), C.c_size_t(self.cap), C.byref(nv), C.c_void_p(self.indices.ptr),
+                C.c_size_t(self.cap), C.byref(ni), C.c_void_p(self.colors.ptr), C.c_void_p(self.ub["endpoints"].ptr),
+                C.c_void_p(self.ub["seed_steps"].ptr), C.byref(steps)), "round")
+            if ncont.value:
+                raise RuntimeError("segment region overflow: %d seeds need a continuation pass" % ncont.value)
+            return dict(particles=n.value, steps=int(steps.value), vertices=nv.value, indices=ni.value)
         r = self.tracer.trace_union(parts, self.region, buffers=self.ub, order=self.order)
         if r["continuation"].n:
             raise RuntimeError("segment region overflow: %d seeds need a continuation pass" % r["continuation"].n)
         m = C.c_size_t()
-        if self.order == "chunked":
+        if self.order == "group8":
+            ctx.check(lib.pa_gpu_prune_counted_group8(
+                ctx.handle, C.c_void_p(self.ub["vertices"].ptr), C.c_size_t(self.region),
+                C.c_void_p(self.ub["counts"].ptr), n, C.c_void_p(self.pruned.ptr), C.c_size_t(self.cap),
+                C.byref(m), C.c_void_p(self.offs.ptr)), "prune")
+        elif self.order == "chunked":
             ctx.check(lib.pa_gpu_prune_counted_chunked(
                 ctx.handle, C.c_void_p(self.ub["vertices"].ptr), C.c_size_t(self.region),
                 C.c_void_p(self.ub["counts"].ptr), n, C.c_void_p(self.pruned.ptr), C.c_size_t(self.cap),
@@ -111,8 +129,8 @@ class Pipeline:
         return dict(particles=n.value, steps=r["total_steps"], vertices=m.value, indices=ni.value)
 
 
-def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order, engine=None):
-    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order, engine)
+def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order, engine=None, fused=False):
+    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order, engine, fused)
     for _ in range(warmup):
         info = p.step()
     ctx.synchronize()
@@ -131,7 +149,8 @@ def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, or
     ctx.synchronize()
     wall = time.perf_counter() - t0
     ctx.timing(False)
-    kt = {k: ctx.kernel_time(k) for k in ("seeds", "trace_union", "prune_scan", "prune_counted", "index", "color")}
+    kt = {k: ctx.kernel_time(k) for k in ("seeds", "trace_union", "prune_scan", "prune_counted", "index", "color",
+                                          "round_counts", "round_finish")}
     trace_ms = kt["trace_union"][0] / max(kt["trace_union"][1], 1)
     return p, dict(wall_s=wall, ms_per_step=1e3 * wall / steps, steps_per_pass=total // steps,
                    pass_latency_ms=dict(p50=1e3 * float(np.percentile(lat, 50)),
@@ -232,7 +251,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--forms", default="fp32-cells,fp32,bf16-cells,bf16,fp16-cells,fp32-cells-consistent,"
-                                       "fp32-cells-lane")
+                                       "fp32-cells-stages,fp32-cells-lane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
@@ -277,9 +296,12 @@ def main():
                                                   time.perf_counter() - t0))
         interp = pa.INTERP_CONSISTENT if "consistent" in form.split("-") else pa.INTERP_REFERENCE
         tokens = form.split("-")
-        order = "seed" if "seedmajor" in tokens else "chunked" if "quad" in tokens else "step"
-        engine = pa.ENGINE_LANE if "lane" in tokens else pa.ENGINE_HYBRID if order == "step" else None
-        p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order, engine)
+        order = "seed" if "seedmajor" in tokens else "chunked" if "quad" in tokens else \
+            "step" if "lane" in tokens else "group8"
+        engine = pa.ENGINE_LANE if "lane" in tokens else pa.ENGINE_HYBRID if order == "group8" else None
+        fused = engine == pa.ENGINE_HYBRID and "stages" not in tokens
+        p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order, engine,
+                          fused)
         log("%s: %.3f ms/step, %.3e steps/s, trace kernel %.3f ms, %s" %
             (form, res["ms_per_step"], res["value"], res["trace_ms"], res["info"]))
         results[form] = res

@@ -242,7 +242,7 @@ pa_status   pa_gpu_tracer_set_interpolation(pa_gpu_tracer* tracer, pa_interp int
  * particle over the four lanes of a quad, one vec4 component each, which
  * shortens the serial step chain that the longest trajectories of a round
  * sit on; persistent waves refill quads from a work counter.
- * PA_ENGINE_HYBRID (union trace only, step-major slots) runs lane mode while
+ * PA_ENGINE_HYBRID (union trace only, grouped slots) runs lane mode while
  * a wave holds more than 16 live particles and moves the survivors onto
  * quads after that.  All engines produce identical results;
  * PA_ENGINE_LANE is the default.  waves = resident waves of the cooperative
@@ -309,26 +309,53 @@ pa_status   pa_gpu_tracer_trace_union(pa_gpu_tracer* tracer, const pa_particle* 
  * lanes of a wave step in lockstep, so every vertex store is one contiguous
  * run, which is what the memory system rewards.  Pack either into the
  * reference's pruned layout with pa_gpu_prune_counted /
- * pa_gpu_prune_counted_step_major / pa_gpu_prune_counted_chunked.
- * Step-major needs PA_ENGINE_LANE, chunked PA_ENGINE_QUAD. */
+ * pa_gpu_prune_counted_step_major / pa_gpu_prune_counted_chunked /
+ * pa_gpu_prune_counted_group8.  Step-major needs PA_ENGINE_LANE, chunked
+ * PA_ENGINE_QUAD, grouped PA_ENGINE_HYBRID. */
 typedef enum pa_slot_order {
   PA_SLOTS_SEED_MAJOR = 0,
   PA_SLOTS_STEP_MAJOR = 1,
   /* seed-major 8-slot chunks, component-planar: slot s of seed p is float
    * (p * region_slots + s - s % 8) * 4 + 8 * c + s % 8 for component c;
    * region_slots a multiple of 8; written by PA_ENGINE_QUAD only. */
-  PA_SLOTS_CHUNKED = 2
+  PA_SLOTS_CHUNKED = 2,
+  /* 8-seed groups, step-major inside a group: slot s of seed p is vec4
+   * ((p / 8) * region_slots + s) * 8 + p % 8 (allocate ceil(n / 8) * 8 *
+   * region_slots vec4); written by PA_ENGINE_HYBRID only. */
+  PA_SLOTS_GROUP8 = 3
 } pa_slot_order;
 pa_status   pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
                                              size_t region_slots, pa_slot_order order, float* d_vertices,
                                              int32_t* d_vertex_counts, pa_particle* d_continuation,
                                              size_t* n_continuation, float* d_endpoints, int32_t* d_seed_steps,
                                              uint64_t* total_steps);
+/* Extension: one whole round of the union trace, fused: trace_union on the
+ * hybrid engine into grouped d_slots (PA_SLOTS_GROUP8, ceil(n / 8) * 8 *
+ * region_slots vec4, scratch for the caller), then prune + index_generator +
+ * color_generator of those slots.  d_vertices / d_indices / d_colors receive
+ * exactly what pa_gpu_prune_counted_group8, pa_gpu_index_generate and
+ * pa_gpu_color_generate would produce on them (colours of vertices that no
+ * segment colours are left untouched).  Replaces the stage sequence
+ * particle_tracer::trace (pars/source/pipeline.cpp:186-189) -> prune
+ * (:213-220) -> index_generator (:224-234) -> color_generator (:238-260)
+ * for one rank's share traced over its 7 preloaded blocks. */
+pa_status   pa_gpu_tracer_trace_union_round(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
+                                            size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
+                                            pa_particle* d_continuation, size_t* n_continuation,
+                                            pa_color_mode mode, float parameter, float* d_vertices,
+                                            size_t vertex_capacity, size_t* n_vertices, int32_t* d_indices,
+                                            size_t index_capacity, size_t* n_indices, float* d_colors,
+                                            float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps);
 /* Counted prune of step-major slots (row stride n seeds) into the pruned
  * particle-major layout; same outputs as pa_gpu_prune_counted. */
 pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots, size_t n,
                                             const int32_t* d_vertex_counts, float* d_out, size_t capacity,
                                             size_t* n_out, int64_t* d_offsets);
+/* Counted prune of 8-seed grouped slots (PA_SLOTS_GROUP8) into the pruned
+ * layout. */
+pa_status   pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
+                                        const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,
+                                        size_t* n_out, int64_t* d_offsets);
 /* Counted prune of chunked slots (PA_SLOTS_CHUNKED) into the pruned layout. */
 pa_status   pa_gpu_prune_counted_chunked(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
                                          const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,

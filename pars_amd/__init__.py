@@ -52,8 +52,9 @@ STOP_ISZERO, STOP_NORM = 0, 1
 INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
 LAYOUT_NODES, LAYOUT_CELLS = 0, 1
 ENGINE_LANE, ENGINE_COOPERATIVE, ENGINE_QUAD, ENGINE_HYBRID = 0, 1, 2, 3
-SLOTS_SEED_MAJOR, SLOTS_STEP_MAJOR, SLOTS_CHUNKED = 0, 1, 2
-SLOT_ORDERS = {"seed": SLOTS_SEED_MAJOR, "step": SLOTS_STEP_MAJOR, "chunked": SLOTS_CHUNKED}
+SLOTS_SEED_MAJOR, SLOTS_STEP_MAJOR, SLOTS_CHUNKED, SLOTS_GROUP8 = 0, 1, 2, 3
+SLOT_ORDERS = {"seed": SLOTS_SEED_MAJOR, "step": SLOTS_STEP_MAJOR, "chunked": SLOTS_CHUNKED,
+               "group8": SLOTS_GROUP8}
 TERMINATION_VERTEX = np.array([-2.0, -2.0, -2.0, 0.0], np.float32)
 INVALID_VERTEX = np.array([-1.0, -1.0, -1.0, 0.0], np.float32)
 INVALID_INDEX = -1
@@ -446,7 +447,8 @@ class particle_tracer:
         """Union-of-7 trace; buffers (from union_buffers) may be reused across calls.
 
         order "seed": slot s of seed p at p * region_slots + s; "step": at s * n + p;
-        "chunked" (quad engine): 8-slot component-planar chunks per seed."""
+        "chunked" (quad engine): 8-slot component-planar chunks per seed; "group8" (hybrid
+        engine): at ((p // 8) * region_slots + s) * 8 + p % 8."""
         b = buffers or self.union_buffers(particles.n, region_slots, endpoints)
         ncont = C.c_size_t()
         steps = C.c_uint64()
@@ -458,9 +460,34 @@ class particle_tracer:
         return dict(vertices=b["vertices"], counts=b["counts"], continuation=Particles(b["cont"], ncont.value),
                     endpoints=b.get("endpoints"), seed_steps=b.get("seed_steps"), total_steps=int(steps.value))
 
+    def trace_union_round(self, particles: Particles, region_slots: int, mode: int = 2, parameter: float = 0.75,
+                          buffers=None, colors: DeviceBuffer = None):
+        """One fused round: union trace (hybrid engine, grouped slots) + prune + index + colour.
+
+        Returns the outputs of the three stages (vertices, indices, colours) besides the trace's."""
+        n = particles.n
+        b = buffers or self.union_buffers(n, region_slots, endpoints=True)
+        vcap = max(1, n * region_slots)
+        if "out_vertices" not in b:
+            b["out_vertices"] = self.ctx.alloc(vcap * 16)
+            b["out_indices"] = self.ctx.alloc(vcap * 4)
+            b["out_colors"] = self.ctx.alloc(vcap * 16)
+        col = colors if colors is not None else b["out_colors"]
+        ncont, nv, ni, steps = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint64()
+        self.ctx.check(lib().pa_gpu_tracer_trace_union_round(
+            self._h, C.c_void_p(particles.ptr), C.c_size_t(n), C.c_size_t(region_slots),
+            C.c_void_p(b["vertices"].ptr), C.c_void_p(b["counts"].ptr), C.c_void_p(b["cont"].ptr), C.byref(ncont),
+            C.c_int(mode), C.c_float(parameter), C.c_void_p(b["out_vertices"].ptr), C.c_size_t(vcap), C.byref(nv),
+            C.c_void_p(b["out_indices"].ptr), C.c_size_t(vcap), C.byref(ni), C.c_void_p(col.ptr),
+            C.c_void_p(b["endpoints"].ptr), C.c_void_p(b["seed_steps"].ptr), C.byref(steps)), "trace_union_round")
+        return dict(slots=b["vertices"], counts=b["counts"], continuation=Particles(b["cont"], ncont.value),
+                    vertices=b["out_vertices"], n_vertices=nv.value, indices=b["out_indices"], n_indices=ni.value,
+                    colors=col, endpoints=b["endpoints"], seed_steps=b["seed_steps"], total_steps=int(steps.value))
+
     def union_buffers(self, n: int, region_slots: int, endpoints: bool = False):
         n1 = max(n, 1)
-        b = dict(vertices=self.ctx.alloc(n1 * region_slots * 16), counts=self.ctx.alloc(n1 * 4),
+        n8 = (n1 + 7) // 8 * 8  # the grouped order rounds the seeds up to groups of 8
+        b = dict(vertices=self.ctx.alloc(n8 * region_slots * 16), counts=self.ctx.alloc(n1 * 4),
                  cont=self.ctx.alloc(n1 * 48))
         if endpoints:
             b["endpoints"] = self.ctx.alloc(n1 * 16)
@@ -495,6 +522,18 @@ class particle_tracer:
         ctx.check(lib().pa_gpu_prune_counted(ctx.handle, C.c_void_p(slots), C.c_size_t(stride_slots),
                                              C.c_void_p(counts.ptr), C.c_size_t(n), C.c_void_p(out.ptr),
                                              C.c_size_t(capacity), C.byref(m), C.c_void_p(offs.ptr)), "prune_counted")
+        return out, m.value, offs
+
+    @staticmethod
+    def prune_counted_group8(ctx: Context, slots: int, region_slots: int, counts: DeviceBuffer, n: int,
+                             capacity: int, out: DeviceBuffer = None, offs: DeviceBuffer = None):
+        """Pack 8-seed grouped union slots (hybrid engine) into the pruned particle-major layout."""
+        out = ctx.alloc(max(capacity, 1) * 16) if out is None else out
+        offs = ctx.alloc((n + 1) * 8) if offs is None else offs
+        m = C.c_size_t()
+        ctx.check(lib().pa_gpu_prune_counted_group8(
+            ctx.handle, C.c_void_p(slots), C.c_size_t(region_slots), C.c_void_p(counts.ptr), C.c_size_t(n),
+            C.c_void_p(out.ptr), C.c_size_t(capacity), C.byref(m), C.c_void_p(offs.ptr)), "prune_counted_group8")
         return out, m.value, offs
 
     @staticmethod

@@ -328,6 +328,12 @@ __global__ void k_bucket_totals(const int64_t* __restrict__ offs, int64_t nb, in
 
 }  // namespace
 
+size_t scan_workspace_bytes(int64_t n) { return scan_ws_total(n); }
+
+pa_status scan_i32(pa_gpu_ctx* ctx, const int32_t* d_in, int64_t n, int64_t* d_out, int64_t* ws) {
+  return scan_rec<int32_t>(ctx, d_in, n, d_out, ws);
+}
+
 pa_status exclusive_scan_i64(pa_gpu_ctx* ctx, const int64_t* d_in, int64_t* d_out, size_t n) {
   void* ws = nullptr;
   PA_TRY(scratch(ctx, scan_ws_total(int64_t(n)), &ws));

@@ -38,6 +38,11 @@ struct FieldDesc {
   int32_t fast_div;      // spacing within [2^-20, 2^20] on every axis
 };
 
+// Vec4 index of slot s of seed p in 8-seed grouped slots (PA_SLOTS_GROUP8).
+__host__ __device__ __forceinline__ int64_t group8_slot(int64_t p, int64_t s, int64_t region) {
+  return ((p >> 3) * region + s) * 8 + (p & 7);
+}
+
 // Per-launch tracer configuration.  Slot 0 is the own block (vector field
 // index -1), slots 1..6 the neighbours 0..5.
 struct TraceConfig {
@@ -138,8 +143,23 @@ pa_status launch_quad_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, c
                             unsigned long long* steps, unsigned* ncont, unsigned long long* work);
 // Union trace on the hybrid engine (lane mode, then quad mode) into
 // step-major slots.
+// pairs (optional): per-seed index-generator pairs inside the seed's run.
 pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
-                              unsigned long long* steps, unsigned* ncont);
+                              unsigned long long* steps, unsigned* ncont, int32_t* pairs = nullptr);
+// Exclusive scan of n int32 values into n+1 int64 outputs with the caller's
+// workspace of scan_workspace_bytes(n) bytes.
+size_t scan_workspace_bytes(int64_t n);
+pa_status scan_i32(pa_gpu_ctx* ctx, const int32_t* d_in, int64_t n, int64_t* d_out, int64_t* ws);
+// Prune + index + colour of grouped union slots (the rest of a fused
+// round); pairs = per-seed in-run index pairs from the hybrid tracer.
+pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_t region, const int32_t* d_counts,
+                        const int32_t* d_pairs, int mode, float param, float* d_vertices, size_t vertex_capacity,
+                        size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
+                        float* d_colors, void* ws);
+size_t finish_ws_bytes(int64_t n);
+// The colour stage's kernel (one lane per index).
+pa_status color_launch(pa_gpu_ctx* ctx, const float4* d_vertices, const int32_t* d_indices, int64_t n_indices,
+                       int mode, float param, float4* d_colors);
 
 // Exclusive scan of n int64 values in place on the device (n+1 outputs:
 // out[n] = total).  Uses ctx scratch.

@@ -16,6 +16,13 @@ pa_status group_handoffs(pa_gpu_ctx* ctx, const int32_t* d_code, const pa_partic
                          pa_handoff* handoff, void* ws);
 
 namespace {
+pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_particles, size_t n,
+                           size_t region_slots, pa_slot_order order, float* d_vertices, int32_t* d_vertex_counts,
+                           pa_particle* d_continuation, size_t* n_continuation, float* d_endpoints,
+                           int32_t* d_seed_steps, uint64_t* total_steps, int32_t* pairs);
+}  // namespace
+
+namespace {
 
 constexpr int kBlock = 256;
 
@@ -708,6 +715,51 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
                                            size_t* n_continuation, float* d_endpoints, int32_t* d_seed_steps,
                                            uint64_t* total_steps) {
   if (!t) return PA_E_INVALID;
+  return trace_union_impl(t, t->engine, d_particles, n, region_slots, order, d_vertices, d_vertex_counts,
+                          d_continuation, n_continuation, d_endpoints, d_seed_steps, total_steps, nullptr);
+}
+
+pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n,
+                                          size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
+                                          pa_particle* d_continuation, size_t* n_continuation, pa_color_mode mode,
+                                          float parameter, float* d_vertices, size_t vertex_capacity,
+                                          size_t* n_vertices, int32_t* d_indices, size_t index_capacity,
+                                          size_t* n_indices, float* d_colors, float* d_endpoints,
+                                          int32_t* d_seed_steps, uint64_t* total_steps) {
+  if (!t || !n_vertices || !n_indices) return PA_E_INVALID;
+  pa_gpu_ctx* ctx = t->ctx;
+  *n_vertices = 0;
+  *n_indices = 0;
+  if (n == 0) {
+    if (n_continuation) *n_continuation = 0;
+    if (total_steps) *total_steps = 0;
+    return PA_OK;
+  }
+  if (!d_vertices || !d_indices || !d_colors) return PA_E_INVALID;
+  if (mode != PA_COLOR_HSL_CONSTANT_S && mode != PA_COLOR_HSL_CONSTANT_L && mode != PA_COLOR_HSV_CONSTANT_S &&
+      mode != PA_COLOR_HSV_CONSTANT_V && mode != PA_COLOR_RGB && mode != PA_COLOR_MAGNITUDE)
+    return fail(ctx, PA_E_INVALID, "unknown color mode");
+  if (n > size_t(INT32_MAX)) return fail(ctx, PA_E_INVALID, "round: at most 2^31-1 seeds per call");
+  void* ws = nullptr;
+  const size_t pair_bytes = (n * sizeof(int32_t) + 255) / 256 * 256;
+  PA_TRY(scratch(ctx, pair_bytes + finish_ws_bytes(int64_t(n)), &ws));
+  auto* pairs = static_cast<int32_t*>(ws);
+  PA_TRY(trace_union_impl(t, PA_ENGINE_HYBRID, d_particles, n, region_slots, PA_SLOTS_GROUP8, d_slots,
+                          d_vertex_counts, d_continuation, n_continuation, d_endpoints, d_seed_steps, total_steps,
+                          pairs));
+  return finish_group8(ctx, d_slots, int64_t(n), int64_t(region_slots), d_vertex_counts, pairs, int(mode), parameter,
+                       d_vertices, vertex_capacity, n_vertices, d_indices, index_capacity, n_indices, d_colors,
+                       static_cast<char*>(ws) + pair_bytes);
+}
+
+}  // extern "C"
+
+namespace pa_amd {
+namespace {
+pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_particles, size_t n,
+                           size_t region_slots, pa_slot_order order, float* d_vertices, int32_t* d_vertex_counts,
+                           pa_particle* d_continuation, size_t* n_continuation, float* d_endpoints,
+                           int32_t* d_seed_steps, uint64_t* total_steps, int32_t* pairs) {
   pa_gpu_ctx* ctx = t->ctx;
   if (n_continuation) *n_continuation = 0;
   if (n == 0) {
@@ -723,13 +775,14 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
   if (maxr < 0) return fail(ctx, PA_E_INVALID, "negative remaining_iterations");
   if (region_slots < size_t(maxr < 1 ? 1 : maxr))
     return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
-  if (order != PA_SLOTS_SEED_MAJOR && order != PA_SLOTS_STEP_MAJOR && order != PA_SLOTS_CHUNKED)
+  if (order != PA_SLOTS_SEED_MAJOR && order != PA_SLOTS_STEP_MAJOR && order != PA_SLOTS_CHUNKED &&
+      order != PA_SLOTS_GROUP8)
     return fail(ctx, PA_E_INVALID, "unknown slot order");
-  if (order == PA_SLOTS_STEP_MAJOR && t->engine != PA_ENGINE_LANE && t->engine != PA_ENGINE_HYBRID)
-    return fail(ctx, PA_E_UNSUPPORTED, "step-major slots need the lane or hybrid engine");
-  if (t->engine == PA_ENGINE_HYBRID && order != PA_SLOTS_STEP_MAJOR)
-    return fail(ctx, PA_E_UNSUPPORTED, "the hybrid engine writes step-major slots");
-  if ((order == PA_SLOTS_CHUNKED) != (t->engine == PA_ENGINE_QUAD))
+  if (order == PA_SLOTS_STEP_MAJOR && engine != PA_ENGINE_LANE)
+    return fail(ctx, PA_E_UNSUPPORTED, "step-major slots need the lane engine");
+  if ((order == PA_SLOTS_GROUP8) != (engine == PA_ENGINE_HYBRID))
+    return fail(ctx, PA_E_UNSUPPORTED, "the hybrid engine writes (only) 8-seed grouped slots");
+  if ((order == PA_SLOTS_CHUNKED) != (engine == PA_ENGINE_QUAD))
     return fail(ctx, PA_E_UNSUPPORTED, "the quad engine writes (only) chunked slots");
   const int64_t seed_stride = order == PA_SLOTS_STEP_MAJOR ? 1 : int64_t(region_slots);
   const int64_t slot_stride = order == PA_SLOTS_STEP_MAJOR ? int64_t(n) : 1;
@@ -737,7 +790,7 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
   auto* steps = reinterpret_cast<unsigned long long*>(cbase + 1024);
   auto* ncont = reinterpret_cast<unsigned*>(cbase + 1032);
   const int mode = mode_of(t->integrator);
-  if (t->engine == PA_ENGINE_COOPERATIVE || t->engine == PA_ENGINE_QUAD || t->engine == PA_ENGINE_HYBRID) {
+  if (engine == PA_ENGINE_COOPERATIVE || engine == PA_ENGINE_QUAD || engine == PA_ENGINE_HYBRID) {
     CoopLaunch L;
     L.particles = d_particles;
     L.n = int64_t(n);
@@ -749,13 +802,13 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
     L.cont = d_continuation;
     L.endpoints = reinterpret_cast<float4*>(d_endpoints);
     L.seed_steps = d_seed_steps;
-    if (t->engine == PA_ENGINE_HYBRID) {
+    if (engine == PA_ENGINE_HYBRID) {
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 2);
       launch_begin(ctx, "trace_union");
-      const pa_status st = launch_hybrid_union(ctx, cfg, mode, L, steps, ncont);
+      const pa_status st = launch_hybrid_union(ctx, cfg, mode, L, steps, ncont, pairs);
       launch_end(ctx);
       if (st != PA_OK) return st;
-    } else if (t->engine == PA_ENGINE_QUAD) {
+    } else if (engine == PA_ENGINE_QUAD) {
       auto* work = reinterpret_cast<unsigned long long*>(cbase + 1040);
       hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, steps, 3);
       launch_begin(ctx, "trace_union");
@@ -785,6 +838,10 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
   *n_continuation = size_t(unsigned(h[1] & 0xffffffffu));
   return PA_OK;
 }
+}  // namespace
+}  // namespace pa_amd
+
+extern "C" {
 
 pa_status pa_gpu_advect(pa_gpu_tracer* t, pa_particle* d_particles, size_t n, int32_t* d_state,
                         int32_t* d_handoff_slot, uint64_t* total_steps) {

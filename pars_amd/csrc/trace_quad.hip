@@ -21,6 +21,7 @@
 // hand-offs as in the lane engine), bit for bit.
 #include <hip/hip_runtime.h>
 
+#include "pa_color.hpp"
 #include "pa_device.hpp"
 
 namespace pa_amd {
@@ -246,6 +247,7 @@ struct QState {
   int state;
   int64_t pid;
   int b, R, it, slot, nsteps, orank;
+  int pairs, prevT;  // index-generator pairs so far, last slot was the termination vertex
 };
 
 // A continuation record for a seed whose region is full (as the lane engine).
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
   st.state = Q_IDLE;
   st.pid = 0;
   st.b = -1;
-  st.R = st.it = st.slot = st.nsteps = st.orank = 0;
+  st.R = st.it = st.slot = st.nsteps = st.orank = st.pairs = st.prevT = 0;
   float last = 0.0f;
   QLane L;
   load_lane(tab, blk, -1, c, L);
@@ -414,8 +416,9 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
   if (lane == 0 && wave_steps) atomicAdd(steps_out, wave_steps);
 }
 
-// Hybrid union trace, step-major slots (slot s of seed p = vec4 at s * n + p,
-// as the lane engine's PA_SLOTS_STEP_MAJOR).  Every lane starts with one seed
+// Hybrid union trace into 8-seed grouped slots (PA_SLOTS_GROUP8: slot s of
+// seed p = vec4 at ((p / 8) * region + s) * 8 + p % 8; a wave's stores of one
+// slot fill whole 128-B lines, and a group's slots are contiguous).  Every lane starts with one seed
 // (lane mode, all four components per lane: the most particles in flight
 // while the round is throughput-bound).  Once a wave has at most 16 live
 // particles, which is where the longest trajectories leave it latency-bound,
@@ -432,11 +435,12 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
                                                              unsigned int* __restrict__ n_cont,
                                                              unsigned long long* __restrict__ steps_out, int rule,
                                                              float4* __restrict__ endpoints,
-                                                             int32_t* __restrict__ seed_steps) {
+                                                             int32_t* __restrict__ seed_steps,
+                                                             int32_t* __restrict__ pairs_out) {
   __shared__ QEntry tab[7][4];
   __shared__ QBlock blk[7];
   __shared__ float4 x_last[kQuadBlock / 64][kQuadSwitch];
-  __shared__ int32_t x_int[kQuadBlock / 64][kQuadSwitch][8];
+  __shared__ int32_t x_int[kQuadBlock / 64][kQuadSwitch][10];
   fill_tables<S>(cfg, tab, blk);
   __syncthreads();
 
@@ -447,7 +451,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   st.pid = logical_block(cfg.xcd_remap) * kQuadBlock + threadIdx.x;
   st.state = st.pid < n ? Q_START : Q_IDLE;
   st.b = -1;
-  st.R = st.it = st.slot = st.nsteps = st.orank = 0;
+  st.R = st.it = st.slot = st.nsteps = st.orank = st.pairs = st.prevT = 0;
   float4 last = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   FieldDesc f = cfg.field[0];
   if (st.state == Q_START) {
@@ -511,10 +515,14 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
           }
         }
       }
-      verts[int64_t(st.slot - 1) * n + st.pid] = val;
+      verts[group8_slot(st.pid, st.slot - 1, region)] = val;
+      const bool isT = is_termination_vertex(val);
+      if (st.slot > 1 && !st.prevT && !isT) ++st.pairs;
+      st.prevT = isT;
       if (st.state == Q_DONE) {
         counts[st.pid] = st.slot;
         if (seed_steps) seed_steps[st.pid] = st.nsteps;
+        if (pairs_out) pairs_out[st.pid] = st.pairs;
         if (endpoints) endpoints[st.pid] = last;
         wave_steps += static_cast<unsigned long long>(st.nsteps);
         st.state = Q_IDLE;
@@ -537,6 +545,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       xi[5] = st.slot;
       xi[6] = st.nsteps;
       xi[7] = st.orank;
+      xi[8] = st.pairs;
+      xi[9] = st.prevT;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -558,6 +568,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       qs.slot = xi[5];
       qs.nsteps = xi[6];
       qs.orank = xi[7];
+      qs.pairs = xi[8];
+      qs.prevT = xi[9];
       load_lane(tab, blk, qs.b, c, L);
     }
     float* vf = reinterpret_cast<float*>(verts);
@@ -565,11 +577,15 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       if (__ballot(qs.state != Q_IDLE) == 0) break;
       if (qs.state != Q_IDLE) {
         const float val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont);
-        vf[(int64_t(qs.slot - 1) * n + qs.pid) * 4 + c] = val;
+        vf[group8_slot(qs.pid, qs.slot - 1, region) * 4 + c] = val;
+        const bool isT = quad_all(val == (c == 3 ? 0.0f : -2.0f));
+        if (qs.slot > 1 && !qs.prevT && !isT) ++qs.pairs;
+        qs.prevT = isT;
         if (qs.state == Q_DONE) {
           if (c == 0) {
             counts[qs.pid] = qs.slot;
             if (seed_steps) seed_steps[qs.pid] = qs.nsteps;
+            if (pairs_out) pairs_out[qs.pid] = qs.pairs;
             wave_steps += static_cast<unsigned long long>(qs.nsteps);
           }
           if (endpoints) reinterpret_cast<float*>(endpoints)[qs.pid * 4 + c] = ql;
@@ -585,13 +601,13 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
 }  // namespace
 
 pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
-                              unsigned long long* steps, unsigned* ncont) {
+                              unsigned long long* steps, unsigned* ncont, int32_t* pairs) {
   if (l.n > INT32_MAX) return fail(ctx, PA_E_INVALID, "hybrid engine: at most 2^31-1 seeds per call");
   const int64_t blocks = (l.n + kQuadBlock - 1) / kQuadBlock;
   dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
     hipLaunchKernelGGL((k_hybrid_union<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
                        dim3(unsigned(blocks)), dim3(kQuadBlock), 0, ctx->stream, cfg, l.particles, l.n, l.verts,
-                       l.region, l.counts, l.cont, ncont, steps, l.stop_rule, l.endpoints, l.seed_steps);
+                       l.region, l.counts, l.cont, ncont, steps, l.stop_rule, l.endpoints, l.seed_steps, pairs);
   }, cfg.field[0].layout);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;

@@ -144,7 +144,7 @@ def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout, engine):
 @pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
 @pytest.mark.parametrize("engine,order", [(pa.ENGINE_COOPERATIVE, "seed"), (pa.ENGINE_LANE, "seed"),
                                           (pa.ENGINE_LANE, "step"), (pa.ENGINE_QUAD, "chunked"),
-                                          (pa.ENGINE_HYBRID, "step")])
+                                          (pa.ENGINE_HYBRID, "group8")])
 @pytest.mark.parametrize("integ", [pa.RUNGE_KUTTA_4, pa.CLASSIC_RK4])
 def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, order, integ):
     """Union-of-7 continuation vs the oracle's round emulation: every seed's
@@ -169,8 +169,11 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, o
         r = t.trace_union(pending, region, order=order)
         steps += r["total_steps"]
         cnt = r["counts"].download(np.int32, pending.n)
-        v = r["vertices"].download(np.float32, pending.n * region * 4)
-        if order == "step":
+        n8 = (pending.n + 7) // 8 * 8
+        v = r["vertices"].download(np.float32, (n8 if order == "group8" else pending.n) * region * 4)
+        if order == "group8":
+            v = v.reshape(n8 // 8, region, 8, 4).transpose(0, 2, 1, 3).reshape(n8, region, 4)[:pending.n]
+        elif order == "step":
             v = v.reshape(region, pending.n, 4).transpose(1, 0, 2)
         elif order == "chunked":  # [seed][chunk][component][8]
             v = v.reshape(pending.n, region // 8, 4, 8).transpose(0, 1, 3, 2).reshape(pending.n, region, 4)
@@ -185,6 +188,9 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, o
         elif order == "chunked":
             packed, m, offs = pa.particle_tracer.prune_counted_chunked(ctx, r["vertices"].ptr, region, r["counts"],
                                                                        pending.n, total)
+        elif order == "group8":
+            packed, m, offs = pa.particle_tracer.prune_counted_group8(ctx, r["vertices"].ptr, region, r["counts"],
+                                                                      pending.n, total)
         else:
             packed, m, offs = pa.particle_tracer.prune_counted(ctx, r["vertices"].ptr, region, r["counts"],
                                                                pending.n, total)
@@ -210,6 +216,49 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, o
         mine = np.concatenate(got[i])
         assert mine.tobytes() == want.tobytes(), i
     assert (ou["segment_block"] >= 0).any()  # some seeds did cross into neighbours
+
+
+@pytest.mark.parametrize("storage,layout", [(pa.FP32, pa.LAYOUT_CELLS), (pa.BF16, pa.LAYOUT_NODES)])
+@pytest.mark.parametrize("mode", [2, 0, 4, 100])
+def test_union_round_fused(ctx, orc, storage, layout, mode):
+    """The fused round (union trace + prune + index + colour in one call)
+    equals the separate stages on the same slots, bit for bit, including
+    seeds that end without a termination vertex (remaining_iterations 1)."""
+    S = ShareFields(ctx, orc, 64, 64, 21, storage, layout=layout)
+    sh = S.share
+    dt, R = 0.02, 300
+    seeds = orc.seed_generate(sh.local.offset, sh.local.size, sh.local.spacing * f32(2), R, 21)
+    seeds["remaining_iterations"][::97] = 1  # curves of a single vertex: index pairs cross into the next seed
+    seeds["remaining_iterations"][5::211] = 0
+    n = len(seeds)
+    parts = pa.Particles.from_host(ctx, seeds)
+    region = R + 40
+    t = S.tracer(ctx, dt, pa.RUNGE_KUTTA_4, pa.INTERP_REFERENCE, pa.ENGINE_HYBRID)
+    sentinel = np.full(n * region * 4, 7.5, np.float32)
+    col_fused = ctx.upload(sentinel)
+    r = t.trace_union_round(parts, region, mode, 0.6, colors=col_fused)
+    assert r["continuation"].n == 0
+    # separate stages on the same slots
+    t2 = S.tracer(ctx, dt, pa.RUNGE_KUTTA_4, pa.INTERP_REFERENCE, pa.ENGINE_HYBRID)
+    u = t2.trace_union(parts, region, order="group8")
+    assert u["total_steps"] == r["total_steps"]
+    cnt = u["counts"].download(np.int32, n)
+    assert np.array_equal(cnt, r["counts"].download(np.int32, n))
+    total = int(cnt.sum())
+    packed, m, _ = pa.particle_tracer.prune_counted_group8(ctx, u["vertices"].ptr, region, u["counts"], n, total)
+    assert r["n_vertices"] == m == total
+    assert r["vertices"].download(np.float32, 4 * m).tobytes() == packed.download(np.float32, 4 * m).tobytes()
+    idx, ni = pa.index_generator.generate(ctx, packed, m)
+    assert r["n_indices"] == ni
+    assert np.array_equal(r["indices"].download(np.int32, ni), idx.download(np.int32, ni))
+    col_sep = ctx.upload(sentinel)
+    ctx.check(pa.lib().pa_gpu_color_generate(ctx.handle, pa.C.c_void_p(packed.ptr), pa.C.c_size_t(m),
+                                             pa.C.c_void_p(idx.ptr), pa.C.c_size_t(ni), pa.C.c_int(mode),
+                                             pa.C.c_float(0.6), pa.C.c_void_p(col_sep.ptr)), "color")
+    assert col_fused.download(np.float32, 4 * m).tobytes() == col_sep.download(np.float32, 4 * m).tobytes()
+    # and the index list is the oracle's on the packed vertices
+    oidx = orc.index_generate(packed.download(np.float32, 4 * m).reshape(-1, 4))
+    assert np.array_equal(idx.download(np.int32, ni), oidx)
 
 
 def test_prune_index_color_parity(ctx, orc):
