@@ -49,6 +49,8 @@ INTEGRATORS = {
     "classic_rk4": CLASSIC_RK4,
 }
 STOP_ISZERO, STOP_NORM = 0, 1
+COLOR_HSL_CONSTANT_S, COLOR_HSL_CONSTANT_L, COLOR_HSV_CONSTANT_S, COLOR_HSV_CONSTANT_V, COLOR_RGB = range(5)
+COLOR_MAGNITUDE = 100
 INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
 LAYOUT_NODES, LAYOUT_CELLS = 0, 1
 ENGINE_LANE, ENGINE_COOPERATIVE, ENGINE_QUAD, ENGINE_HYBRID = 0, 1, 2, 3
