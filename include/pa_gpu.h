@@ -361,6 +361,40 @@ pa_status   pa_gpu_prune_counted_chunked(pa_gpu_ctx* ctx, const float* d_slots, 
                                          const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,
                                          size_t* n_out, int64_t* d_offsets);
 
+/* Extension: particle pool over up to 16 resident fields.  It serves the
+ * batched form (many seed sets over fields of uneven size in one persistent
+ * work-queue kernel) and is the step engine of the serving loop (bounded
+ * passes, requests admitted between passes).  No reference counterpart:
+ * pars serves one request at a time (pars/source/service.cpp:20-69).  Each
+ * particle follows the reference round (particle_tracer.cpp:265-326) on
+ * its own field alone (field index = its vector_field_index; no
+ * neighbours, stop on leaving the field): the endpoint and step count equal
+ * a one-block trace's last vertex and step count; vertices are not kept.
+ * A particle's result does not depend on the rest of the pool. */
+enum { PA_POOL_ACTIVE = 0, PA_POOL_DONE = 1, PA_POOL_INVALID = 2 };
+typedef struct pa_pool {
+  float*   position;     /* n vec4: seed position in, endpoint out */
+  int32_t* iteration;    /* the reference loop counter it (starts at 1) */
+  int32_t* remaining;    /* remaining_iterations R */
+  int32_t* steps;        /* integrator steps taken so far */
+  int32_t* status;       /* PA_POOL_* */
+  int32_t* field;        /* field index */
+  int32_t* request;      /* optional: request id (the particle's original_rank) */
+  int32_t* request_done; /* optional: finished particles per request id (zeroed by the caller) */
+} pa_pool;
+/* Fill pool entries [0, n) from particles; entries whose field index is not
+ * in [0, n_fields) become PA_POOL_INVALID and the call returns PA_E_INVALID
+ * after initialising the rest. */
+pa_status   pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
+                             const pa_pool* pool);
+/* Advance every active entry of [0, n) by at most max_iterations reference
+ * iterations (0 = to the end) with the tracer's integrator, step size, stop
+ * rule and interpolation; waves_per_cu = resident waves of the persistent
+ * kernel per CU (0 = 16).  All fields share one storage and layout. */
+pa_status   pa_gpu_pool_run(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
+                            const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
+                            uint64_t* total_steps);
+
 /* prune (particle_tracer.cpp:402-409): order-preserving removal of
  * invalid_vertex from one chunk.  *n_out receives the kept count. */
 pa_status   pa_gpu_prune(pa_gpu_ctx* ctx, const float* d_vertices, size_t n, float* d_out, size_t* n_out);

@@ -563,6 +563,62 @@ class particle_tracer:
         return out, m.value, offs
 
 
+POOL_ACTIVE, POOL_DONE, POOL_INVALID = 0, 1, 2
+
+
+class _PoolC(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in ("position", "iteration", "remaining", "steps", "status", "field",
+                                                 "request", "request_done")]
+
+
+class ParticlePool:
+    """Particle pool over up to 16 resident fields (pa_gpu_pool_*): batched seed sets and the
+    serving loop's step engine.  Each entry follows a one-block reference trace on its field."""
+
+    def __init__(self, ctx: Context, capacity: int, requests: int = 0):
+        self.ctx, self.capacity = ctx, capacity
+        c = max(capacity, 1)
+        self.position = ctx.alloc(c * 16)
+        self.iteration = ctx.alloc(c * 4)
+        self.remaining = ctx.alloc(c * 4)
+        self.steps = ctx.alloc(c * 4)
+        self.status = ctx.alloc(c * 4)
+        self.field = ctx.alloc(c * 4)
+        self.request = ctx.alloc(c * 4) if requests else None
+        self.request_done = ctx.upload(np.zeros(max(requests, 1), np.int32)) if requests else None
+        self.n = 0
+
+    def _c(self, offset: int = 0):
+        def at(b, size):
+            return C.c_void_p(b.ptr + offset * size) if b is not None else None
+        return _PoolC(at(self.position, 16), at(self.iteration, 4), at(self.remaining, 4), at(self.steps, 4),
+                      at(self.status, 4), at(self.field, 4), at(self.request, 4),
+                      C.c_void_p(self.request_done.ptr) if self.request_done is not None else None)
+
+    def init(self, particles: "Particles", n_fields: int, offset: int = 0):
+        """Entries [offset, offset + particles.n) from particles (field = vector_field_index,
+        request = original_rank)."""
+        assert offset + particles.n <= self.capacity
+        pc = self._c(offset)
+        self.ctx.check(lib().pa_gpu_pool_init(self.ctx.handle, C.c_void_p(particles.ptr), C.c_size_t(particles.n),
+                                              C.c_int32(n_fields), C.byref(pc)), "pool_init")
+        self.n = max(self.n, offset + particles.n)
+
+    def run(self, tracer: "particle_tracer", fields, n: int = None, max_iterations: int = 0, waves_per_cu: int = 0):
+        arr = (C.c_void_p * len(fields))(*[f.handle for f in fields])
+        steps = C.c_uint64()
+        pc = self._c()
+        self.ctx.check(lib().pa_gpu_pool_run(tracer._h, arr, C.c_int32(len(fields)), C.byref(pc),
+                                             C.c_size_t(self.n if n is None else n), C.c_int32(max_iterations),
+                                             C.c_int32(waves_per_cu), C.byref(steps)), "pool_run")
+        return int(steps.value)
+
+    def results(self, n: int = None):
+        n = self.n if n is None else n
+        return dict(position=self.position.download(np.float32, 4 * n).reshape(n, 4),
+                    steps=self.steps.download(np.int32, n), status=self.status.download(np.int32, n))
+
+
 class index_generator:
     """pa::index_generator (index_generator.cpp:10-24)."""
 

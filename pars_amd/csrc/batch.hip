@@ -1,0 +1,209 @@
+// Particle pool over up to 16 resident fields: the batched form (many seed
+// sets over fields of uneven size in one persistent work-queue kernel) and
+// the step engine of the serving loop (bounded passes, requests admitted
+// between passes).  No reference counterpart (service.cpp:20-69 serves one
+// request at a time); every particle follows the reference round of
+// particle_tracer.cpp:265-326 on its own field alone: it = 1 .. R-1, stop on
+// leaving the field, on the stop rule or at it == R-1, so its endpoint and
+// step count are those of a one-block reference trace (the vertices are not
+// kept).  A particle's result does not depend on what else is in the pool.
+#include <cstring>
+
+#include "pa_device.hpp"
+
+namespace pa_amd {
+namespace {
+
+constexpr int kPoolBlock = 256;
+constexpr int kMaxFields = 16;
+
+struct PoolConfig {
+  TraceConfig k;                // integrator constants, stop rule, interpolation
+  FieldDesc f[kMaxFields];      // field i is particle.vector_field_index == i
+  int32_t n_fields;
+};
+
+__global__ __launch_bounds__(kPoolBlock) void k_pool_init(const pa_particle* __restrict__ P, int64_t n, int n_fields,
+                                                          float4* __restrict__ pos, int32_t* __restrict__ it,
+                                                          int32_t* __restrict__ rem, int32_t* __restrict__ steps,
+                                                          int32_t* __restrict__ status, int32_t* __restrict__ field,
+                                                          int32_t* __restrict__ request,
+                                                          unsigned int* __restrict__ invalid) {
+  const int64_t i = int64_t(blockIdx.x) * kPoolBlock + threadIdx.x;
+  if (i >= n) return;
+  const pa_particle p = P[i];
+  pos[i] = make_float4(p.position[0], p.position[1], p.position[2], p.position[3]);
+  it[i] = 1;
+  rem[i] = p.remaining_iterations;
+  steps[i] = 0;
+  const bool ok = p.vector_field_index >= 0 && p.vector_field_index < n_fields;
+  status[i] = ok ? PA_POOL_ACTIVE : PA_POOL_INVALID;
+  field[i] = ok ? p.vector_field_index : 0;
+  if (request) request[i] = p.original_rank;
+  if (!ok) atomicAdd(invalid, 1u);
+}
+
+// Persistent: lanes take pool entries from a work counter (one atomic per
+// wave per refill), advance each for at most `budget` iterations (or to the
+// end), store the state back and take the next.  Finished entries add one to
+// their request's counter.
+template <int S, int MODE, int INTERP>
+__global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
+                                                         int32_t* __restrict__ it_a, const int32_t* __restrict__ rem,
+                                                         int32_t* __restrict__ steps_a,
+                                                         int32_t* __restrict__ status_a,
+                                                         const int32_t* __restrict__ field_a,
+                                                         const int32_t* __restrict__ request,
+                                                         int32_t* __restrict__ request_done, int64_t n, int budget,
+                                                         int rule, unsigned long long* __restrict__ work,
+                                                         unsigned long long* __restrict__ steps_out) {
+  const int lane = threadIdx.x & 63;
+  bool have = false, open = true;
+  int64_t i = 0;
+  float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int it = 0, R = 0, st = 0, left = 0;
+  FieldDesc f = cfg.f[0];
+  unsigned long long done_steps = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!have);
+    if (open && idle) {
+      const uint32_t take = uint32_t(__popcll(idle));
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(work, static_cast<unsigned long long>(take));
+      const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(base & 0xffffffffu));
+      const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(base >> 32));
+      base = (static_cast<unsigned long long>(hi) << 32) | lo;
+      if (base + take >= static_cast<unsigned long long>(n)) open = false;
+      if (!have) {
+        const unsigned long long idx = base + uint32_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
+        if (idx < static_cast<unsigned long long>(n) && status_a[idx] == PA_POOL_ACTIVE) {
+          i = int64_t(idx);
+          p = pos[i];
+          it = it_a[i];
+          R = rem[i];
+          st = steps_a[i];
+          f = cfg.f[field_a[i]];
+          left = budget;
+          have = true;
+        }
+      }
+    }
+    if (__ballot(have) == 0) {
+      if (!open) break;
+      continue;
+    }
+    if (have) {
+      bool fin = it >= R - 1;  // the reference breaks at it == R-1 (R < 2: the seed alone)
+      if (!fin) {
+        float4 next;
+        if (advance_rt<S, MODE, INTERP>(cfg.k, f, p, next, rule) != 0) {
+          fin = true;
+        } else {
+          p = next;
+          ++it;
+          ++st;
+          ++done_steps;
+        }
+      }
+      if (fin || --left == 0) {
+        pos[i] = p;
+        it_a[i] = it;
+        steps_a[i] = st;
+        if (fin) {
+          status_a[i] = PA_POOL_DONE;
+          if (request && request_done) atomicAdd(&request_done[request[i]], 1);
+        }
+        have = false;
+      }
+    }
+  }
+  done_steps = wave_sum_u64(done_steps);
+  if (lane == 0 && done_steps) atomicAdd(steps_out, done_steps);
+}
+
+__global__ void k_pool_zero(unsigned long long* c) {
+  if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+}
+
+}  // namespace
+}  // namespace pa_amd
+
+using namespace pa_amd;
+
+extern "C" {
+
+pa_status pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
+                           const pa_pool* pool) {
+  if (!ctx || !pool) return PA_E_INVALID;
+  if (n == 0) return PA_OK;
+  if (!d_particles || !pool->position || !pool->iteration || !pool->remaining || !pool->steps || !pool->status ||
+      !pool->field)
+    return PA_E_INVALID;
+  if (n_fields < 1 || n_fields > kMaxFields) return fail(ctx, PA_E_INVALID, "pool: 1 to 16 fields");
+  auto* invalid = reinterpret_cast<unsigned int*>(static_cast<char*>(ctx->counters) + 2080);
+  hipLaunchKernelGGL(k_pool_zero, dim3(1), dim3(64), 0, ctx->stream,
+                     reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2080));
+  launch_begin(ctx, "pool_init");
+  hipLaunchKernelGGL(k_pool_init, dim3(unsigned((n + kPoolBlock - 1) / kPoolBlock)), dim3(kPoolBlock), 0, ctx->stream,
+                     d_particles, int64_t(n), int(n_fields), reinterpret_cast<float4*>(pool->position),
+                     pool->iteration, pool->remaining, pool->steps, pool->status, pool->field, pool->request, invalid);
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  unsigned bad = 0;
+  PA_HIP(ctx, hipMemcpyAsync(&bad, invalid, sizeof bad, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (bad) return fail(ctx, PA_E_INVALID, "pool: particles with a field index outside [0, n_fields)");
+  return PA_OK;
+}
+
+pa_status pa_gpu_pool_run(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32_t n_fields, const pa_pool* pool,
+                          size_t n, int32_t max_iterations, int32_t waves_per_cu, uint64_t* total_steps) {
+  if (!t || !pool) return PA_E_INVALID;
+  pa_gpu_ctx* ctx = t->ctx;
+  if (total_steps) *total_steps = 0;
+  if (n == 0) return PA_OK;
+  if (!fields || n_fields < 1 || n_fields > kMaxFields) return fail(ctx, PA_E_INVALID, "pool: 1 to 16 fields");
+  if (max_iterations < 0 || waves_per_cu < 0) return fail(ctx, PA_E_INVALID, "pool: negative budget or waves");
+  PA_TRY(check_integrator_of(ctx, t->integrator));
+  PoolConfig cfg;
+  std::memset(&cfg, 0, sizeof cfg);
+  fill_step_constants(t, &cfg.k);
+  int code = -1;
+  for (int k = 0; k < n_fields; ++k) {
+    if (!fields[k]) return fail(ctx, PA_E_INVALID, "pool: null field");
+    const int c = fields[k]->desc.storage + 3 * fields[k]->desc.layout;
+    if (code >= 0 && c != code)
+      return fail(ctx, PA_E_UNSUPPORTED, "pool: all fields must share one storage type and layout");
+    code = c;
+    cfg.f[k] = fields[k]->desc;
+  }
+  for (int k = n_fields; k < kMaxFields; ++k) cfg.f[k] = cfg.f[0];
+  cfg.n_fields = n_fields;
+  auto* counters = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2048);
+  int cus = 0;
+  PA_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int per_cu = waves_per_cu > 0 ? waves_per_cu : 16;
+  int64_t blocks = (int64_t(cus) * per_cu + 3) / 4;
+  const int64_t need = (int64_t(n) + kPoolBlock - 1) / kPoolBlock;
+  if (blocks > need) blocks = need;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_pool_zero, dim3(1), dim3(64), 0, ctx->stream, counters);
+  const int budget = max_iterations > 0 ? max_iterations : INT32_MAX;
+  launch_begin(ctx, "pool_run");
+  dispatch(cfg.f[0].storage, mode_of_integrator(t->integrator), cfg.k.interp, [&](auto S, auto M, auto I) {
+    hipLaunchKernelGGL((k_pool_run<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
+                       dim3(unsigned(blocks)), dim3(kPoolBlock), 0, ctx->stream, cfg,
+                       reinterpret_cast<float4*>(pool->position), pool->iteration, pool->remaining, pool->steps,
+                       pool->status, pool->field, pool->request, pool->request_done, int64_t(n), budget,
+                       t->stop_rule, counters, counters + 1);
+  }, cfg.f[0].layout);
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  unsigned long long s = 0;
+  PA_HIP(ctx, hipMemcpyAsync(&s, counters + 1, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (total_steps) *total_steps = s;
+  return PA_OK;
+}
+
+}  // extern "C"

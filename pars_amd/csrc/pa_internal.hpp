@@ -117,6 +117,10 @@ void launch_begin(pa_gpu_ctx* ctx, const char* name);
 void launch_end(pa_gpu_ctx* ctx);
 // Build the per-launch configuration from a tracer.
 pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg);
+// Integrator constants, stop rule and interpolation of a tracer (no fields).
+void fill_step_constants(const pa_gpu_tracer* t, TraceConfig* cfg);
+int mode_of_integrator(int integrator);
+pa_status check_integrator_of(pa_gpu_ctx* ctx, int integrator);
 
 // Launch of the wave-cooperative tracer (trace_coop.hip).
 struct CoopLaunch {

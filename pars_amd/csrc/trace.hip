@@ -419,6 +419,11 @@ pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg) {
     if (t->neighbor_rank[k] >= 0) cfg->local_mask |= 1 << k;
     cfg->face_mask[k] = t->face_mask[k];
   }
+  fill_step_constants(t, cfg);
+  return PA_OK;
+}
+
+void fill_step_constants(const pa_gpu_tracer* t, TraceConfig* cfg) {
   const float dt = t->step_size;
   // odeint runge_kutta4 tableau in value_type float: b = {1/6, 1/3, 1/3, 1/6}.
   const float b0 = 1.0f / 6.0f, b1 = 1.0f / 3.0f, b2 = 1.0f / 3.0f, b3 = 1.0f / 6.0f;
@@ -433,7 +438,16 @@ pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg) {
   cfg->stop_eps2 = t->stop_eps * t->stop_eps;
   cfg->interp = t->interp;
   cfg->xcd_remap = t->xcd_remap;
-  return PA_OK;
+}
+
+int mode_of_integrator(int integrator) { return mode_of(integrator); }
+
+pa_status check_integrator_of(pa_gpu_ctx* ctx, int integ) {
+  if (integ == PA_INTEGRATOR_EULER || integ == PA_INTEGRATOR_RUNGE_KUTTA_4 || integ == PA_INTEGRATOR_CLASSIC_RK4)
+    return PA_OK;
+  if (integ >= 0 && integ <= 7)
+    return fail(ctx, PA_E_UNSUPPORTED, "only euler, runge_kutta_4 and classic RK4 run on the GPU");
+  return fail(ctx, PA_E_INVALID, "unknown integrator");
 }
 
 }  // namespace pa_amd

@@ -369,3 +369,96 @@ def test_edge_cases(ctx, orc):
     # index generation of an empty array is rejected (size()-1 underflow)
     with pytest.raises(pa.PaError):
         pa.index_generator.generate(ctx, ctx.alloc(16), 0)
+
+
+def _pool_fields(ctx, orc, storage, layout):
+    """Two fields of different size sharing storage and layout: ABC on 40^3 and
+    a random-Fourier curl field on 48^3 (synthesized on the device)."""
+    from pars_amd import workloads as W
+    d0, o0, _ = single_block(ctx, orc, 40, storage, layout=layout)
+    n = 48
+    h = 2 * np.pi / (n - 1)
+    hf = f32(h)
+    modes = W.fourier_modes(12, 7)
+    off = np.zeros(3, np.float32)
+    size = np.array([f32(n) * hf] * 3, np.float32)
+    sp = np.array([hf] * 3, np.float32)
+    d1 = pa.VectorField.synthesize(ctx, "fourier", modes, (n, n, n), (0, 0, 0), h, off, size, sp,
+                                   storage).set_layout(layout)
+    o1 = orc.Field(d1.read(), off, size, sp)
+    return [d0, d1], [o0, o1]
+
+
+def _pool_particles(orc, fields_o, n_per, seed):
+    rng = np.random.default_rng(seed)
+    parts = []
+    for k, o in enumerate(fields_o):
+        p = np.zeros(n_per, orc.PARTICLE_DTYPE)
+        p["position"][:, :3] = rng.uniform(0.05, 0.95, (n_per, 3)).astype(np.float32) * o.size[0]
+        p["remaining_iterations"] = rng.integers(0, 260, n_per)
+        p["remaining_iterations"][:3] = [0, 1, 2]
+        p["vector_field_index"] = k
+        p["original_rank"] = k  # request id = field
+        parts.append(p)
+    allp = np.concatenate(parts)
+    return allp[rng.permutation(len(allp))]
+
+
+@pytest.mark.parametrize("storage,layout", [(pa.FP32, pa.LAYOUT_NODES), (pa.BF16, pa.LAYOUT_CELLS)])
+def test_pool_batched_parity(ctx, orc, storage, layout):
+    """Batched seed sets over two fields in one persistent kernel: every
+    endpoint and step count equals the oracle's one-block trace; bounded
+    passes and a pool holding one set alone give the same bits."""
+    fields, fo = _pool_fields(ctx, orc, storage, layout)
+    parts = _pool_particles(orc, fo, 700, 3)
+    n = len(parts)
+    t = pa.particle_tracer(ctx)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.01)
+    pool = pa.ParticlePool(ctx, n, requests=2)
+    dev = pa.Particles.from_host(ctx, parts)
+    pool.init(dev, 2)
+    total = pool.run(t, fields)
+    res = pool.results()
+    assert (res["status"] == pa.POOL_DONE).all()
+    want_pos = np.zeros((n, 4), np.float32)
+    want_steps = np.zeros(n, np.int32)
+    for k in range(2):
+        sel = np.nonzero(parts["vector_field_index"] == k)[0]
+        q = parts[sel].copy()
+        q["vector_field_index"] = -1
+        e, st = orc.endpoints(q, fo[k], dt=0.01, integrator=orc.RK4)
+        want_pos[sel], want_steps[sel] = e, st
+    assert res["position"].tobytes() == want_pos.tobytes()
+    assert np.array_equal(res["steps"], want_steps)
+    assert total == int(want_steps.sum())
+    done = pool.request_done.download(np.int32, 2)
+    assert np.array_equal(done, np.bincount(parts["vector_field_index"], minlength=2))
+    # bounded passes of 7 iterations reach the same state
+    pool2 = pa.ParticlePool(ctx, n)
+    pool2.init(dev, 2)
+    passes, total2 = 0, 0
+    while True:
+        total2 += pool2.run(t, fields, max_iterations=7, waves_per_cu=2)
+        passes += 1
+        r2 = pool2.results()
+        if (r2["status"] == pa.POOL_DONE).all():
+            break
+        assert passes < 200
+    assert passes > 5 and total2 == total
+    assert r2["position"].tobytes() == res["position"].tobytes()
+    # one set alone
+    sel = np.nonzero(parts["vector_field_index"] == 1)[0]
+    pool3 = pa.ParticlePool(ctx, len(sel))
+    pool3.init(pa.Particles.from_host(ctx, parts[sel]), 2)
+    pool3.run(t, fields)
+    assert pool3.results()["position"].tobytes() == res["position"][sel].tobytes()
+
+
+def test_pool_rejects_bad_field_index(ctx, orc):
+    fields, fo = _pool_fields(ctx, orc, pa.FP32, pa.LAYOUT_NODES)
+    parts = _pool_particles(orc, fo, 10, 4)
+    parts["vector_field_index"][3] = 2
+    pool = pa.ParticlePool(ctx, len(parts))
+    with pytest.raises(pa.PaError):
+        pool.init(pa.Particles.from_host(ctx, parts), 2)
