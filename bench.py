@@ -237,6 +237,127 @@ def bf16_ratio(forms, extra):
                 trace_ratio=extra[twin]["trace_steps_per_s"] / extra[head]["trace_steps_per_s"])
 
 
+def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01):
+    """FTLE workload: ABC on n^3 nodes (one block), one seed per node, `steps`
+    RK4 steps (advector R = steps + 1), T = steps * dt; flow map in FP32,
+    Jacobian / Cauchy-Green / eigen in FP64.  Parity at full size: sampled
+    flow-map voxels against the oracle's advector (bit-exact), and the FTLE
+    of an interior sub-cube against the oracle's FP64 post-processing of the
+    same flow-map values (1e-4 relative)."""
+    from pars_amd import workloads as W
+
+    h = 2.0 * np.pi / (n - 1)
+    hf = np.float32(h)
+    size = np.array([np.float32(n) * hf] * 3, np.float32)
+    sp = np.array([hf] * 3, np.float32)
+    field = pa.VectorField.synthesize(ctx, "abc", W.ABC, (n, n, n), (0, 0, 0), h, np.zeros(3, np.float32), size, sp,
+                                      pa.FP32)
+    fm = pa.flow_map_generator(ctx)
+    fm.set_vector_field(field)
+    fm.set_integrator(pa.RUNGE_KUTTA_4)
+    fm.set_step_size(dt)
+    T = steps * dt
+    out, dims, spacing, _ = fm.generate((n, n, n), steps + 1, 1.0)  # warm-up
+    ftle = pa.ftle_map_generator(ctx)
+    ctx.synchronize()
+    ctx.timing_reset()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    out, dims, spacing, nsteps = fm.generate((n, n, n), steps + 1, 1.0)
+    ftle.set_flow_map(out, dims, spacing)
+    ftle.set_time(T)
+    fv, _ = ftle.generate()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.timing(False)
+    fms = ctx.kernel_time("flow_map")
+    fts = ctx.kernel_time("ftle")
+    flow_ms = fms[0] / max(fms[1], 1)
+    ftle_ms = fts[0] / max(fts[1], 1)
+    res = dict(workload="ABC %d^3, %d seeds (one per node), %d RK4 steps, dt %g, T %g; FP32 flow map, FP64 FTLE"
+                        % (n, n ** 3, steps, dt, T),
+               steps=nsteps, flow_map_ms=flow_ms, ftle_ms=ftle_ms, wall_ms=1e3 * wall,
+               flow_map_steps_per_s=nsteps / (flow_ms * 1e-3), ftle_nodes_per_s=n ** 3 / (ftle_ms * 1e-3),
+               roofline=dict(bound="hbm", algorithmic_bytes_per_step=96,
+                             achieved_gbs=nsteps * 96 / (flow_ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS))
+    if orc is not None:
+        flow = out.download(np.float32, n ** 3 * 3).reshape(n, n, n, 3)
+        # flow-map parity on sampled voxels
+        rng = np.random.default_rng(5)
+        ijk = rng.integers(0, n, (512, 3))
+        p = np.zeros(len(ijk), orc.PARTICLE_DTYPE)
+        p["position"][:, :3] = (hf * ijk.astype(np.float32)).astype(np.float32)
+        p["remaining_iterations"] = steps + 1
+        p["vector_field_index"] = -1
+        of = orc.Field(field.read(), np.zeros(3, np.float32), size, sp)
+        q, _, _, _ = orc.advect(p, of, dt=dt, integrator=orc.RK4)
+        got = flow[ijk[:, 0], ijk[:, 1], ijk[:, 2]]
+        res["flow_map_sample_bitexact"] = bool(got.tobytes() == q["position"][:, :3].copy().tobytes())
+        # FP64 post-processing of an interior sub-cube
+        c0, m = n // 2 - 16, 34
+        sub = flow[c0:c0 + m, c0:c0 + m, c0:c0 + m]
+        of_, _ = orc.ftle(sub, sp, T)
+        gpu = fv.download(np.float32, n ** 3).reshape(n, n, n)[c0 + 1:c0 + m - 1, c0 + 1:c0 + m - 1,
+                                                              c0 + 1:c0 + m - 1]
+        want = of_[1:-1, 1:-1, 1:-1]
+        rel = np.abs(gpu.astype(np.float64) - want) / np.maximum(np.abs(want), 1e-3)
+        res["ftle_max_rel_err_vs_oracle"] = float(rel.max())
+    field.release()
+    return res
+
+
+def bench_batched(pa, ctx, cache=None):
+    """Batched uneven work: 8 random-Fourier curl fields (64^3 .. 256^3, BF16),
+    64 seed sets (log-uniform 1e3..1e6 seeds, 100..2000 steps, dt 0.01) in one
+    persistent work-queue kernel, launch shape from the autotuner."""
+    from pars_amd import batch as B
+
+    wl = B.make_workload()
+    tun = B.autotune(ctx, wl, cache=cache)
+    fields = B.build_fields(ctx, wl, pa.BF16, tun.layout)
+    steps, sec, pool = B.run_batch(ctx, wl, fields, tun.waves_per_cu)  # warm-up
+    steps, sec, pool = B.run_batch(ctx, wl, fields, tun.waves_per_cu, pool=pool)
+    done = pool.request_done.download(np.int32, len(wl.sets))
+    complete = bool(all(done[k] == s["size"] for k, s in enumerate(wl.sets)))
+    for f in fields:
+        f.release()
+    return dict(workload="8 curl fields of 48 random modes (64^3..256^3 nodes, BF16), 64 seed sets "
+                         "(%d seeds), 100..2000 RK4 steps per set, dt 0.01, one persistent launch" % wl.n,
+                particles=wl.n, steps=steps, seconds=sec, steps_per_s=steps / sec, all_sets_complete=complete,
+                tuning=dict(layout="cells" if tun.layout == pa.LAYOUT_CELLS else "nodes",
+                            waves_per_cu=tun.waves_per_cu, sample_rates=tun.table))
+
+
+def bench_serving(pa, ctx, rates=(200.0, 400.0, 800.0), n_requests=60):
+    """Serving: one 1024^3 BF16 random-Fourier field (64 modes) resident;
+    Poisson arrivals; requests of 256..262144 seeds in boxes of 1-10% of the
+    domain; 500 RK4 steps, dt 0.002.  Continuous batching against one
+    request at a time on an FP32 copy of the field."""
+    from pars_amd import serve as S
+
+    f16 = S.make_field(ctx, pa.BF16)
+    f32 = S.make_field(ctx, pa.FP32)
+    server = S.Server(ctx, [f16], pass_iterations=50)
+    out = dict(workload="1024^3 BF16 curl field of 64 random modes (8.6 GB) resident; Poisson arrivals; "
+                        "256..262144 seeds per request in 1-10%% boxes; 500 RK4 steps, dt 0.002; "
+                        "%d requests per rate" % n_requests, rates={})
+    warm = S.make_requests(4, 1000.0, seed=1)
+    server.serve(warm, keep_endpoints=False)
+    for rate in rates:
+        reqs = S.make_requests(n_requests, rate, seed=int(rate))
+        steps, wall = server.serve(reqs, keep_endpoints=False)
+        base = S.make_requests(n_requests, rate, seed=int(rate))
+        bsteps, bwall = S.serve_one_at_a_time(ctx, [f32], base, keep_endpoints=False)
+        out["rates"]["%g/s" % rate] = dict(
+            continuous=dict(steps_per_s=steps / wall, requests_per_s=n_requests / wall, **S.latency_stats(reqs)),
+            one_at_a_time_fp32=dict(steps_per_s=bsteps / bwall, requests_per_s=n_requests / bwall,
+                                    **S.latency_stats(base)))
+        log("serving %g/s: %s" % (rate, out["rates"]["%g/s" % rate]))
+    f16.release()
+    f32.release()
+    return out
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -253,6 +374,7 @@ def main():
     ap.add_argument("--forms", default="fp32-cells,fp32,bf16-cells,bf16,fp16-cells,fp32-cells-consistent,"
                                        "fp32-cells-stages,fp32-cells-lane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the FTLE, batched and serving workloads")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -358,6 +480,17 @@ def main():
         cpu = cpu_baseline(orc, share, 1, args.cpu_threads)
         log("cpu baseline:", cpu)
 
+    workloads = {}
+    if not args.no_workloads:
+        for name, fn in (("ftle", lambda: bench_ftle(pa, ctx, orc)), ("batched", lambda: bench_batched(pa, ctx)),
+                         ("serving", lambda: bench_serving(pa, ctx))):
+            t0 = time.perf_counter()
+            try:
+                workloads[name] = fn()
+            except Exception as e:  # reported in the line; the headline stands on its own
+                workloads[name] = dict(error="%s: %s" % (type(e).__name__, e))
+            log("%s (%.1f s): %s" % (name, time.perf_counter() - t0, workloads[name]))
+
     line = dict(
         metric=METRIC, value=base["value"], unit="particle-steps/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=base["ms_per_step"], higher_is_better=True, scaling="weak",
@@ -371,7 +504,7 @@ def main():
                     seeds=n_seeds, remaining_iterations=R, dt=DT, field="ABC A=sqrt3 B=sqrt2 C=1",
                     form=forms[0], device=dev["name"]),
         pass_latency_ms=base["pass_latency_ms"],
-        roofline=roofline, cpu_baseline=cpu, forms=extra,
+        roofline=roofline, cpu_baseline=cpu, forms=extra, workloads=workloads,
         bf16_vs_fp32=bf16_ratio(forms, extra),
     )
     print(json.dumps(line), flush=True)

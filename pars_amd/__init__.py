@@ -586,6 +586,7 @@ class ParticlePool:
         self.field = ctx.alloc(c * 4)
         self.request = ctx.alloc(c * 4) if requests else None
         self.request_done = ctx.upload(np.zeros(max(requests, 1), np.int32)) if requests else None
+        self.status.upload(np.full(c, POOL_INVALID, np.int32))  # entries never initialised are skipped
         self.n = 0
 
     def _c(self, offset: int = 0):

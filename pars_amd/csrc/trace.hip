@@ -68,7 +68,21 @@ __global__ __launch_bounds__(kBlock) void k_max_remaining(const pa_particle* __r
     lo = min(lo, __shfl_xor(lo, o, 64));
     hi = max(hi, __shfl_xor(hi, o, 64));
   }
+  // one atomic per workgroup and value (the counters are shared by the grid)
+  __shared__ int red[3][kBlock / 64];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    red[0][w] = m;
+    red[1][w] = lo;
+    red[2][w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / 64; ++k) {
+      m = max(m, red[0][k]);
+      lo = min(lo, red[1][k]);
+      hi = max(hi, red[2][k]);
+    }
     atomicMax(out_max, m);
     atomicMin(out_min_vfi, lo);
     atomicMax(out_max_vfi, hi);
@@ -376,7 +390,7 @@ pa_status scan_particles(pa_gpu_ctx* ctx, const TraceConfig& cfg, const pa_parti
   int* c = static_cast<int*>(ctx->counters);
   hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(1), 0, ctx->stream, c);
   int64_t g = (int64_t(n) + kBlock - 1) / kBlock;
-  if (g > 4096) g = 4096;
+  if (g > 1024) g = 1024;
   hipLaunchKernelGGL(k_max_remaining, dim3(unsigned(g)), dim3(kBlock), 0, ctx->stream, d_particles, int64_t(n), c,
                      c + 1, c + 2);
   PA_HIP(ctx, hipGetLastError());

@@ -462,3 +462,29 @@ def test_pool_rejects_bad_field_index(ctx, orc):
     pool = pa.ParticlePool(ctx, len(parts))
     with pytest.raises(pa.PaError):
         pool.init(pa.Particles.from_host(ctx, parts), 2)
+
+
+def test_serving_alone_equals_mix(ctx, orc):
+    """Continuous batching: requests that arrive while others run share the
+    pool passes, and every request's endpoints equal its result alone."""
+    from pars_amd import serve as S
+    grid = 64
+    field = S.make_field(ctx, pa.BF16, grid=grid, n_modes=8)
+    reqs = S.make_requests(12, rate=20000.0, grid=grid, steps=120, size_range=(64, 4000))
+    server = S.Server(ctx, [field], capacity=30000, pass_iterations=16, max_in_flight=64)
+    steps, wall = server.serve(reqs)
+    assert all(np.isfinite(r.done) for r in reqs)
+    assert steps == sum(r.steps for r in reqs)
+    overlapped = sum(1 for a in reqs for b in reqs if a is not b and a.admitted < b.admitted < a.done)
+    assert overlapped > 0  # some requests joined while others were on the GPU
+    alone = [S.Request(r.rid, 0.0, r.particles) for r in reqs]
+    S.serve_one_at_a_time(ctx, [field], alone)
+    for r, a in zip(reqs, alone):
+        assert r.endpoints.tobytes() == a.endpoints.tobytes(), r.rid
+        assert r.steps == a.steps
+    # and the pool's one-block trace is the oracle's on the decoded field
+    o = orc.Field(field.read(), field.offset, field.size, field.spacing)
+    q = reqs[0].particles.copy()
+    q["vector_field_index"] = -1
+    e, st = orc.endpoints(q, o, dt=S.DT, integrator=orc.RK4)
+    assert reqs[0].endpoints.tobytes() == e.tobytes()
