@@ -328,19 +328,20 @@ def bench_batched(pa, ctx, cache=None):
                             waves_per_cu=tun.waves_per_cu, sample_rates=tun.table))
 
 
-def bench_serving(pa, ctx, rates=(200.0, 400.0, 800.0), n_requests=60):
+def bench_serving(pa, ctx, rates=(200.0, 400.0, 800.0, 1600.0), n_requests=60):
     """Serving: one 1024^3 BF16 random-Fourier field (64 modes) resident;
     Poisson arrivals; requests of 256..262144 seeds in boxes of 1-10% of the
     domain; 500 RK4 steps, dt 0.002.  Continuous batching against one
     request at a time on an FP32 copy of the field."""
     from pars_amd import serve as S
 
-    f16 = S.make_field(ctx, pa.BF16)
+    f16 = S.make_field(ctx, pa.BF16).set_layout(pa.LAYOUT_CELLS)
     f32 = S.make_field(ctx, pa.FP32)
-    server = S.Server(ctx, [f16], pass_iterations=50)
-    out = dict(workload="1024^3 BF16 curl field of 64 random modes (8.6 GB) resident; Poisson arrivals; "
-                        "256..262144 seeds per request in 1-10%% boxes; 500 RK4 steps, dt 0.002; "
-                        "%d requests per rate" % n_requests, rates={})
+    server = S.Server(ctx, [f16], pass_iterations=200)
+    out = dict(workload="1024^3 BF16 curl field of 64 random modes (8.6 GB nodes, served from 64-B cell records) "
+                        "resident; Poisson arrivals; 256..262144 seeds per request in 1-10%% boxes; 500 RK4 steps, "
+                        "dt 0.002; %d requests per rate; baseline: one request at a time on the FP32 node field"
+                        % n_requests, rates={})
     warm = S.make_requests(4, 1000.0, seed=1)
     server.serve(warm, keep_endpoints=False)
     for rate in rates:

@@ -48,11 +48,13 @@ class BatchWorkload:
 
 
 def make_workload(seed: int = W.DEFAULT_SEED, grids=GRIDS, n_sets=N_SETS, n_modes=N_MODES, size_range=SET_SIZE,
-                  steps_range=STEPS) -> BatchWorkload:
+                  steps_range=STEPS, locality: bool = True) -> BatchWorkload:
     """Fields u = curl A with n_modes random modes each (fourier_modes), on
     grids^3 nodes over [0, 2 pi]^3; n_sets seed sets, sizes log-uniform in
     size_range, each on a uniformly chosen field with positions uniform in
-    its domain, steps uniform in steps_range (R = steps + 2)."""
+    its domain, steps uniform in steps_range (R = steps + 2).  With
+    `locality`, the seeds are stored in locality_order (results are per
+    particle, so the order changes only the speed)."""
     rng = W.SplitMix64(seed)
     modes = [W.fourier_modes(n_modes, seed + 1 + k) for k in range(len(grids))]
     hs = [2.0 * math.pi / (g - 1) for g in grids]
@@ -74,7 +76,30 @@ def make_workload(seed: int = W.DEFAULT_SEED, grids=GRIDS, n_sets=N_SETS, n_mode
         p["vector_field_index"] = fid
         p["original_rank"] = s
         chunks.append(p)
-    return BatchWorkload(grids=grids, modes=modes, sets=sets, particles=np.concatenate(chunks), h=hs)
+    particles = np.concatenate(chunks)
+    if locality:
+        particles = particles[locality_order(particles, hs)]
+    return BatchWorkload(grids=grids, modes=modes, sets=sets, particles=particles, h=hs)
+
+
+def _spread10(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint64) & np.uint64(0x3FF)
+    out = np.zeros_like(x)
+    for b in range(10):
+        out |= ((x >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b)
+    return out
+
+
+def locality_order(particles: np.ndarray, hs: Sequence[float]) -> np.ndarray:
+    """Permutation sorting particles by field, then by the Morton code of
+    their start cell: neighbouring lanes of the pool kernel then gather
+    neighbouring cells of the same field (shared cache lines)."""
+    fid = particles["vector_field_index"].astype(np.int64)
+    h = np.asarray(hs, np.float64)[fid]
+    cell = np.clip(np.floor(particles["position"][:, :3] / h[:, None]), 0, 1023).astype(np.uint64)
+    code = (_spread10(cell[:, 0]) << np.uint64(2)) | (_spread10(cell[:, 1]) << np.uint64(1)) | _spread10(cell[:, 2])
+    key = (fid.astype(np.uint64) << np.uint64(30)) | code
+    return np.argsort(key, kind="stable")
 
 
 def build_fields(ctx: Context, wl: BatchWorkload, storage=BF16, layout=LAYOUT_NODES) -> List[VectorField]:

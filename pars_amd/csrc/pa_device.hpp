@@ -83,7 +83,7 @@ __device__ __forceinline__ float div_corrected(float a, float h, float y) {
 
 __device__ __forceinline__ bool div_range(float a) {
   const float x = fabsf(a);
-  return (x == 0.0f) | ((x >= 0x1p-40f) & (x <= 0x1p40f));
+  return (int(x == 0.0f) | (int(x >= 0x1p-40f) & int(x <= 0x1p40f))) != 0;
 }
 
 // Exact fmodf(d, h) for h > 0 and a located point (0 <= q = d/h rounded <
@@ -106,7 +106,8 @@ __device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, f
   d[0] = px - f.offset[0];
   d[1] = py - f.offset[1];
   d[2] = pz - f.offset[2];
-  const bool fast = (f.fast_div != 0) & div_range(d[0]) & div_range(d[1]) & div_range(d[2]);
+  // bitwise on purpose: no short-circuit branches
+  const bool fast = (int(f.fast_div != 0) & int(div_range(d[0])) & int(div_range(d[1])) & int(div_range(d[2]))) != 0;
   if (__builtin_expect(fast, 1)) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) q[i] = div_corrected(d[i], f.spacing[i], f.inv_spacing[i]);
@@ -141,11 +142,11 @@ __device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3
       w[i] = q[i] - fl;
     } else {
       r[i] = exact_fmod(d[i], f.spacing[i], q[i]);
-      fast = fast & div_range(r[i]);
+      fast = (int(fast) & int(div_range(r[i]))) != 0;
     }
   }
   if constexpr (INTERP != PA_INTERP_CONSISTENT) {
-    if (__builtin_expect(fast & (f.fast_div != 0), 1)) {
+    if (__builtin_expect((int(fast) & int(f.fast_div != 0)) != 0, 1)) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) w[i] = div_corrected(r[i], f.spacing[i], f.inv_spacing[i]);
     } else {

@@ -24,6 +24,7 @@ import numpy as np
 
 from . import (BF16, FP32, Context, ParticlePool, Particles, VectorField, particle_tracer, PARTICLE_DTYPE)
 from . import workloads as W
+from .batch import locality_order
 
 GRID = 1024
 N_MODES = 64
@@ -79,6 +80,7 @@ def make_requests(n: int, rate: float, grid: int = GRID, seed: int = W.DEFAULT_S
         p["position"][:, :3] = (corner + u * side).astype(np.float32)
         p["remaining_iterations"] = steps + 2
         p["vector_field_index"] = 0
+        p = p[locality_order(p, [float(h)])]  # neighbouring lanes trace neighbouring seeds
         out.append(Request(r, t, p))
     return out
 
@@ -141,6 +143,7 @@ class Server:
         ids = list(range(self.max_in_flight))[::-1]       # free request-counter slots
         waiting = list(requests)
         active = {}                                       # counter id -> (request, first slot)
+        left = {}                                         # counter id -> iterations before it surely ends
         steps_total = 0
         t0 = clock()
         while waiting or active:
@@ -162,6 +165,7 @@ class Server:
                 pool.init(Particles.from_host(ctx, p), len(self.fields), offset=a)
                 req.admitted = clock() - t0
                 active[cid] = (req, a)
+                left[cid] = int(p["remaining_iterations"].max())  # iterations until its last particle ends
             if not active:
                 if waiting:
                     wait = waiting[0].arrival - (clock() - t0)
@@ -169,7 +173,15 @@ class Server:
                         time.sleep(min(wait, 0.001))
                 continue
             hw = slots.high_water(self.capacity)
-            steps_total += pool.run(self.tracer, self.fields, n=hw, max_iterations=self.pass_iterations)
+            # pass length: short while requests wait (they join the next pass),
+            # otherwise long enough for the soonest request to finish
+            if waiting and waiting[0].arrival <= clock() - t0:
+                k = self.pass_iterations
+            else:
+                k = max(self.pass_iterations, min(left[c] for c in active))
+            steps_total += pool.run(self.tracer, self.fields, n=hw, max_iterations=k)
+            for c in active:
+                left[c] -= k
             done = pool.request_done.download(np.int32, self.max_in_flight)
             for cid in [c for c, (r, a) in active.items() if done[c] == len(r.particles)]:
                 req, a = active.pop(cid)
@@ -185,8 +197,10 @@ class Server:
 
 def serve_one_at_a_time(ctx: Context, fields: Sequence[VectorField], requests: Sequence[Request],
                         keep_endpoints: bool = True, clock=time.perf_counter):
-    """Baseline: each request alone, in arrival order, one launch to completion."""
+    """Baseline: each request alone, in arrival order, one launch to
+    completion (device buffers allocated once, for the largest request)."""
     t = make_tracer(ctx)
+    pool = ParticlePool(ctx, max(len(r.particles) for r in requests))
     steps_total = 0
     t0 = clock()
     for req in requests:
@@ -195,12 +209,11 @@ def serve_one_at_a_time(ctx: Context, fields: Sequence[VectorField], requests: S
             time.sleep(req.arrival - now)
         req.admitted = clock() - t0
         n = len(req.particles)
-        pool = ParticlePool(ctx, n)
         pool.init(Particles.from_host(ctx, req.particles), len(fields))
-        steps_total += pool.run(t, fields)
-        r = pool.results()
-        req.endpoints = r["position"] if keep_endpoints else None
-        req.steps = int(r["steps"].sum())
+        steps_total += pool.run(t, fields, n=n)
+        if keep_endpoints:
+            req.endpoints = pool.position.download(np.float32, 4 * n).reshape(n, 4)
+        req.steps = int(pool.steps.download(np.int32, n).sum())
         req.done = clock() - t0
     return steps_total, clock() - t0
 
