@@ -7,12 +7,13 @@
 namespace pa_amd {
 namespace {
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_color(const float4* __restrict__ v, const int32_t* __restrict__ idx,
-                                               int64_t n_idx, int mode, float param, float4* __restrict__ colors) {
+                                               int64_t n_idx, float param, float4* __restrict__ colors) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n_idx) return;
   const int64_t k = idx[i];
-  const float4 c = segment_color(v[k], v[k + 1], mode, param);
+  const float4 c = segment_color(v[k], v[k + 1], MODE, param);
   colors[k] = c;
   // The last vertex of a curve gets its segment's colour
   // (color_generator.cpp:36-37); the final index never does.
@@ -26,9 +27,12 @@ pa_status color_launch(pa_gpu_ctx* ctx, const float4* d_vertices, const int32_t*
   if (n_indices == 0) return PA_OK;
   const int64_t blocks = (n_indices + 255) / 256;
   launch_begin(ctx, "color");
-  hipLaunchKernelGGL(k_color, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream, d_vertices, d_indices, n_indices,
-                     mode, param, d_colors);
+  const bool known = with_color_mode(mode, [&](auto M) {
+    hipLaunchKernelGGL(k_color<decltype(M)::value>, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream, d_vertices,
+                       d_indices, n_indices, param, d_colors);
+  });
   launch_end(ctx);
+  if (!known) return fail(ctx, PA_E_INVALID, "unknown color mode");
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;
 }

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pa_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -82,6 +84,21 @@ __device__ __forceinline__ float4 segment_color(float4 a, float4 b, int mode, fl
     t = make_float4(d.x / s, d.y / s, d.z / s, d.w / s);
   }
   return map_color(t, d, mode, param);
+}
+
+// Calls f(std::integral_constant<int, MODE>) for a runtime colour mode, so
+// kernels specialise on it (the other modes' arithmetic folds away).
+template <typename F>
+inline bool with_color_mode(int mode, F&& f) {
+  switch (mode) {
+    case PA_COLOR_HSL_CONSTANT_S: f(std::integral_constant<int, PA_COLOR_HSL_CONSTANT_S>{}); return true;
+    case PA_COLOR_HSL_CONSTANT_L: f(std::integral_constant<int, PA_COLOR_HSL_CONSTANT_L>{}); return true;
+    case PA_COLOR_HSV_CONSTANT_S: f(std::integral_constant<int, PA_COLOR_HSV_CONSTANT_S>{}); return true;
+    case PA_COLOR_HSV_CONSTANT_V: f(std::integral_constant<int, PA_COLOR_HSV_CONSTANT_V>{}); return true;
+    case PA_COLOR_RGB: f(std::integral_constant<int, PA_COLOR_RGB>{}); return true;
+    case PA_COLOR_MAGNITUDE: f(std::integral_constant<int, PA_COLOR_MAGNITUDE>{}); return true;
+    default: return false;
+  }
 }
 
 __device__ __forceinline__ bool is_termination_vertex(float4 a) {

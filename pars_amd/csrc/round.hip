@@ -3,7 +3,7 @@
 // (index_generator.cpp:10-24) and color_generator (color_generator.cpp:
 // 30-98) on 8-seed grouped slots (slot s of seed p = vec4 at
 // ((p / 8) * region + s) * 8 + p % 8): one pass packs the vertices and writes
-// the index list, a dense pass over the indices writes the colours.
+// the index list and the colours.
 //
 // Layout of the outputs (as the separate stages produce them):
 //   vertices  seed p's slots at O[p] .. O[p] + counts[p] - 1, O = exclusive
@@ -41,74 +41,219 @@ __global__ __launch_bounds__(kThreads) void k_round_counts(const float4* __restr
   icount[p] = pairs[p] + x;
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float4 dpp_f4(float4 v) {
+  return make_float4(dpp_f<CTRL>(v.x), dpp_f<CTRL>(v.y), dpp_f<CTRL>(v.z), dpp_f<CTRL>(v.w));
+}
+
+// Output stores are raw buffer stores whose descriptor starts at the group's
+// first output element: a lane with nothing to store gives an offset past
+// the descriptor's range and the hardware drops the store, so every store
+// is issued by the whole wave without branches (exact wait counts) and
+// without traffic of its own.  Offsets are below 2^31 bytes (at most 8
+// seeds of at most 2^24 slots each).
+constexpr int kDrop = 0x7FFFFFF0;
+constexpr int64_t kMaxRegion = int64_t(1) << 24;
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t out_rsrc(void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, kDrop, 0x00020000);
+}
+__device__ __forceinline__ void store16(rsrc_t r, float4 v, int off) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 d = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+  __builtin_amdgcn_raw_buffer_store_b128(d, r, off, 0, 0);
+}
+__device__ __forceinline__ void store4(rsrc_t r, int32_t v, int off) {
+  __builtin_amdgcn_raw_buffer_store_b32(uint32_t(v), r, off, 0, 0);
+}
+
+// A value the whole wave shares, held in scalar registers (the buffer
+// descriptors built from it then need no per-lane loop).
+__device__ __forceinline__ int64_t uniform(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x)));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x) >> 32));
+  return int64_t((uint64_t(hi) << 32) | lo);
+}
+
+// What one lane of k_group8 knows about its seed p (lane 8k + j of the wave
+// holds slot s0 + j of seed k of the group).
+struct SeedLane {
+  int64_t O;         // first packed vertex of the seed
+  int vrel, irel;    // first vertex and first index relative to the group's
+  int c;             // vertex count
+  int j, k, lane;
+  int carry;         // index pairs of this seed before the current chunk
+  int prev_bit7;     // whether slot s0 - 1 begins a pair
+  bool cross;        // the seed's last vertex pairs with the next seed's first
+  bool from_prev;    // slot 0 takes the colour of the previous seed's cross pair
+  float4 first_next; // first vertex of seed p + 1 (when cross)
+  float4 last_prev;  // last vertex of seed p - 1 (when from_prev)
+  float4 prev_tail;  // lane 8k: slot s0 - 1 of the seed
+};
+
+// The group's three outputs (vertices, indices, colours).
+struct GroupOut {
+  rsrc_t v, i, c;
+};
+
+// Colour work of one wave, compacted: segments (a, b) and the vertex that
+// takes their colour are queued in LDS as the rows produce them and coloured
+// 64 at a time, so lanes past a seed's last vertex cost no colour
+// arithmetic.  head and tail are wave-uniform; a row adds at most 64 entries
+// and the queue is drained whenever 64 are waiting, so 128 entries suffice.
+constexpr int kQueue = 128;
+__shared__ float4 q_a[kThreads / 64][kQueue];
+__shared__ float4 q_b[kThreads / 64][kQueue];
+__shared__ int32_t q_dst[kThreads / 64][kQueue];  // vertex relative to the group's first
+struct ColorQueue {
+  int w;  // wave of the workgroup
+  uint32_t head, tail;
+};
+
+// A value select (a select of float4 lvalues would select their addresses
+// and keep the operands in scratch memory).
+__device__ __forceinline__ float4 pick(bool c, float4 x, float4 y) {
+  return make_float4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
+}
+
+template <int MODE>
+__device__ __forceinline__ void color_drain(ColorQueue& q, int lane, uint32_t count, float param,
+                                            const GroupOut& out) {
+  const uint32_t e = (q.tail + uint32_t(lane)) & (kQueue - 1);
+  const float4 col = segment_color(q_a[q.w][e], q_b[q.w][e], MODE, param);
+  store16(out.c, col, uint32_t(lane) < count ? q_dst[q.w][e] * 16 : kDrop);
+  q.tail += count;
+}
+
+// One chunk row (slot s of every seed of the group) of the fused round:
+// the vertex, the index pair (s, s + 1) and the colour of slot s.  Colour
+// rule of color_generator.cpp:24-38 on the packed arrays: a vertex that
+// begins a pair takes that segment's colour; a vertex that does not, but
+// follows one that does, takes the previous segment's colour unless that
+// pair is the last index of the list.
+template <bool ROUND, int MODE>
+__device__ __forceinline__ void finish_row(SeedLane& L, ColorQueue& q, const float4 v, const float4 next_row, int s,
+                                           int64_t NI, int64_t I0, float param, const GroupOut& out) {
+  store16(out.v, v, s < L.c ? (L.vrel + s) * 16 : kDrop);
+  if constexpr (ROUND) {
+    // slot s + 1: the next lane of the seed (DPP row_shl:1); for the last
+    // lane, the first slot of the next chunk (row_shr:7 takes lane 8k to
+    // lane 8k + 7).  Slot s - 1: row_shr:1, for the first lane the tail kept
+    // from the previous chunk.
+    float4 nx = dpp_f4<0x101>(v);
+    const float4 f = dpp_f4<0x117>(next_row);
+    if (L.j == 7) nx = f;
+    float4 pv = dpp_f4<0x111>(v);
+    if (L.j == 0) pv = L.prev_tail;
+    const bool valid = s <= L.c - 2 && !is_termination_vertex(v) && !is_termination_vertex(nx);
+    const uint32_t bits = uint32_t(__ballot(valid) >> (8 * L.k)) & 0xFFu;
+    const int rel = L.irel + L.carry + __popc(bits & ((1u << L.j) - 1u));
+    store4(out.i, int32_t(L.O + s), valid ? rel * 4 : kDrop);
+    const bool prev_valid = L.j ? ((bits >> (L.j - 1)) & 1u) != 0u : L.prev_bit7 != 0;
+    const bool own_cross = !valid && L.cross && s == L.c - 1;
+    const bool after = !valid && !own_cross && s < L.c && prev_valid && I0 + rel != NI;
+    const bool first = !valid && !own_cross && s == 0 && L.from_prev;
+    const bool colour = valid || own_cross || after || first;
+    const uint64_t m = __ballot(colour);
+    if (colour) {
+      const uint32_t e = (q.head + uint32_t(__popcll(m & ((uint64_t(1) << L.lane) - 1u)))) & (kQueue - 1);
+      q_a[q.w][e] = pick(after, pv, pick(first, L.last_prev, v));
+      q_b[q.w][e] = pick(valid, nx, pick(own_cross, L.first_next, v));
+      q_dst[q.w][e] = L.vrel + s;
+    }
+    q.head += uint32_t(__popcll(m));
+    if (q.head - q.tail >= 64u) color_drain<MODE>(q, L.lane, 64u, param, out);
+    L.carry += __popc(bits);
+    L.prev_bit7 = int(bits >> 7);
+    L.prev_tail = dpp_f4<0x107>(v);
+  }
+}
+
 // One wave per group of 8 seeds, whose slots are contiguous: lane 8k + j
 // holds slot s0 + j of seed k, so every load is one 1-KiB run of the group
-// and every seed's 8 vertices (and its valid index pairs) are stored as one
-// contiguous run.  The index position of a pair is the seed's offset, the
-// pairs before this chunk, and the prefix inside the chunk.
-template <bool INDEX>
+// and every seed's 8 vertices (and index pairs, and colours) are stored as
+// one contiguous run.  Two register blocks alternate so the next block's
+// loads are in flight while one is written.  The index position of a pair is
+// the seed's offset, the pairs before this chunk, and the prefix inside it;
+// the pair that crosses into the next seed is the seed's last index.
+template <bool ROUND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_group8(const float4* __restrict__ slots, int64_t n, int64_t region,
                                                      const int32_t* __restrict__ counts,
                                                      const int32_t* __restrict__ cross,
                                                      const int32_t* __restrict__ icount,
                                                      const int64_t* __restrict__ voff,
-                                                     const int64_t* __restrict__ ioff, float4* __restrict__ vout,
-                                                     int32_t* __restrict__ iout) {
+                                                     const int64_t* __restrict__ ioff, int64_t NI, float param,
+                                                     float4* __restrict__ vout, int32_t* __restrict__ iout,
+                                                     float4* __restrict__ cout) {
   const int64_t g = int64_t(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
   if (g * 8 >= n) return;
-  const int lane = threadIdx.x & 63, k = lane >> 3, j = lane & 7;
-  const int64_t p = g * 8 + k;
+  ColorQueue q{int(threadIdx.x >> 6), 0u, 0u};
+  SeedLane L;
+  L.lane = threadIdx.x & 63;
+  L.k = L.lane >> 3;
+  L.j = L.lane & 7;
+  const int64_t p = g * 8 + L.k;
   const bool has = p < n;
-  const int c = has ? counts[p] : 0;
-  int cmax = c;
+  L.c = has ? counts[p] : 0;
+  int cmax = L.c;
 #pragma unroll
   for (int o = 8; o < 64; o <<= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
-  const int64_t O = has ? voff[p] : 0;
-  const int64_t I = (INDEX && has) ? ioff[p] : 0;
+  // the group's first vertex and index (seed 8g, lane 0), wave-uniform
+  const int64_t O0 = uniform(voff[g * 8]);
+  const int64_t I0 = ROUND ? uniform(ioff[g * 8]) : 0;
+  L.O = has ? voff[p] : O0;
+  L.vrel = int(L.O - O0);
+  L.irel = (ROUND && has) ? int(ioff[p] - I0) : 0;
+  GroupOut out;
+  out.v = out_rsrc(vout + O0);
+  out.i = out_rsrc(ROUND ? iout + I0 : nullptr);
+  out.c = out_rsrc(ROUND ? cout + O0 : nullptr);
+  L.carry = 0;
+  L.prev_bit7 = 0;
+  L.cross = ROUND && has && cross[p] != 0;
+  L.from_prev = ROUND && has && p >= 1 && cross[p - 1] != 0 && I0 + L.irel != NI;
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  L.first_next = zero;
+  L.last_prev = zero;
+  L.prev_tail = zero;
+  if (L.cross) L.first_next = slots[group8_slot(p + 1, 0, region)];
+  if (L.from_prev) L.last_prev = slots[group8_slot(p - 1, counts[p - 1] - 1, region)];
   const float4* base = slots + g * region * 8;
-  int carry = 0;
-  // kUnroll chunks of 8 slots per pass: their loads are in flight together
-  for (int s0 = 0; s0 < cmax; s0 += 8 * kUnroll) {
-    float4 v[kUnroll];
+  // slot s of this lane's seed, clamped to the seed's last vertex (rows past
+  // a seed's count then fetch no cache line of their own; their values are
+  // not stored) and into the region
+  const int last = min(max(L.c - 1, 0), int(region - 1));
+  auto load = [&](float4 (&b)[kUnroll], int s0) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int s = s0 + 8 * u + j;
-      v[u] = s < c ? base[int64_t(s) * 8 + k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const int s = s0 + 8 * u + L.j;
+      b[u] = base[int64_t(s < last ? s : last) * 8 + L.k];
     }
-    float4 after = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // slot s0 + 8 kUnroll of this lane's seed
-    if constexpr (INDEX) {
-      const int s = s0 + 8 * kUnroll;
-      if (j == 7 && s < c) after = base[int64_t(s) * 8 + k];
-    }
+  };
+  auto block = [&](const float4 (&b)[kUnroll], const float4& next_first, int s0) {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int s = s0 + 8 * u + j;
-      if (s < c) vout[O + s] = v[u];
-      if constexpr (INDEX) {
-        // slot s + 1: the next lane of this seed, or the first row of the next chunk
-        float4 nx;
-        nx.x = __shfl_down(v[u].x, 1, 64);
-        nx.y = __shfl_down(v[u].y, 1, 64);
-        nx.z = __shfl_down(v[u].z, 1, 64);
-        nx.w = __shfl_down(v[u].w, 1, 64);
-        if (u + 1 < kUnroll) {
-          const int src = 8 * k;
-          const float4 f = make_float4(__shfl(v[u + 1 < kUnroll ? u + 1 : u].x, src, 64),
-                                       __shfl(v[u + 1 < kUnroll ? u + 1 : u].y, src, 64),
-                                       __shfl(v[u + 1 < kUnroll ? u + 1 : u].z, src, 64),
-                                       __shfl(v[u + 1 < kUnroll ? u + 1 : u].w, src, 64));
-          if (j == 7) nx = f;
-        } else if (j == 7) {
-          nx = after;
-        }
-        const bool valid = s <= c - 2 && !is_termination_vertex(v[u]) && !is_termination_vertex(nx);
-        const uint32_t bits = uint32_t(__ballot(valid) >> (8 * k)) & 0xFFu;
-        if (valid) iout[I + carry + __popc(bits & ((1u << j) - 1u))] = int32_t(O + s);
-        carry += __popc(bits);
-      }
-    }
+    for (int u = 0; u < kUnroll; ++u)
+      finish_row<ROUND, MODE>(L, q, b[u], u + 1 < kUnroll ? b[u + 1 < kUnroll ? u + 1 : u] : next_first,
+                              s0 + 8 * u + L.j, NI, I0, param, out);
+  };
+  float4 A[kUnroll], B[kUnroll];
+  load(A, 0);
+  for (int s0 = 0;; s0 += 16 * kUnroll) {
+    load(B, s0 + 8 * kUnroll);
+    block(A, B[0], s0);
+    if (s0 + 8 * kUnroll >= cmax) break;
+    load(A, s0 + 16 * kUnroll);
+    block(B, A[0], s0 + 8 * kUnroll);
+    if (s0 + 16 * kUnroll >= cmax) break;
   }
-  if (INDEX && has && j == 0 && cross[p]) iout[I + icount[p] - 1] = int32_t(O + c - 1);
+  if constexpr (ROUND) {
+    if (q.head != q.tail) color_drain<MODE>(q, L.lane, q.head - q.tail, param, out);
+    if (L.cross && L.j == 0) iout[I0 + L.irel + icount[p] - 1] = int32_t(L.O + L.c - 1);
+  }
 }
 
 }  // namespace
@@ -121,6 +266,7 @@ pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_
                         const int32_t* d_pairs, int mode, float param, float* d_vertices, size_t vertex_capacity,
                         size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
                         float* d_colors, void* ws) {
+  if (region > kMaxRegion) return fail(ctx, PA_E_INVALID, "round: at most 2^24 slots per seed");
   char* w = static_cast<char*>(ws);
   auto* voff = reinterpret_cast<int64_t*>(w);
   auto* ioff = voff + (n + 1);
@@ -147,13 +293,15 @@ pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_
   if (tot[0] > INT32_MAX) return fail(ctx, PA_E_CAPACITY, "round: indices are int32 (at most 2^31-1 vertices)");
   const int64_t groups = (n + 7) / 8;
   launch_begin(ctx, "round_finish");
-  hipLaunchKernelGGL(k_group8<true>, dim3(unsigned((groups + 3) / 4)), dim3(kThreads), 0, ctx->stream, slots, n,
-                     region, d_counts, cross, icount, voff, ioff, reinterpret_cast<float4*>(d_vertices), d_indices);
+  if (!with_color_mode(mode, [&](auto M) {
+        hipLaunchKernelGGL((k_group8<true, decltype(M)::value>), dim3(unsigned((groups + 3) / 4)), dim3(kThreads), 0,
+                           ctx->stream, slots, n, region, d_counts, cross, icount, voff, ioff, tot[1], param,
+                           reinterpret_cast<float4*>(d_vertices), d_indices, reinterpret_cast<float4*>(d_colors));
+      }))
+    return fail(ctx, PA_E_INVALID, "unknown color mode");
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
-  // colours: one lane per index over the packed vertices (dense)
-  return color_launch(ctx, reinterpret_cast<const float4*>(d_vertices), d_indices, tot[1], mode, param,
-                      reinterpret_cast<float4*>(d_colors));
+  return PA_OK;
 }
 
 }  // namespace pa_amd
@@ -168,6 +316,7 @@ extern "C" pa_status pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d
     *n_out = 0;
     return PA_OK;
   }
+  if (region_slots > size_t(kMaxRegion)) return fail(ctx, PA_E_INVALID, "prune_counted: at most 2^24 slots per seed");
   void* ws = nullptr;
   const size_t own = d_offsets ? 0 : (n + 1) * sizeof(int64_t);
   PA_TRY(scratch(ctx, own + scan_workspace_bytes(int64_t(n)) + 4096, &ws));
@@ -183,9 +332,10 @@ extern "C" pa_status pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d
   if (size_t(total) > capacity) return fail(ctx, PA_E_CAPACITY, "prune_counted: output capacity too small");
   const int64_t groups = (int64_t(n) + 7) / 8;
   launch_begin(ctx, "prune_counted");
-  hipLaunchKernelGGL(k_group8<false>, dim3(unsigned((groups + 3) / 4)), dim3(kThreads), 0, ctx->stream,
+  hipLaunchKernelGGL((k_group8<false, 0>), dim3(unsigned((groups + 3) / 4)), dim3(kThreads), 0, ctx->stream,
                      reinterpret_cast<const float4*>(d_slots), int64_t(n), int64_t(region_slots), d_vertex_counts,
-                     nullptr, nullptr, offs, nullptr, reinterpret_cast<float4*>(d_out), nullptr);
+                     nullptr, nullptr, offs, nullptr, int64_t(0), 0.0f, reinterpret_cast<float4*>(d_out),
+                     nullptr, nullptr);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;
