@@ -186,11 +186,16 @@ __global__ __launch_bounds__(kThreads) void k_group8(const float4* __restrict__ 
                                                      const int32_t* __restrict__ cross,
                                                      const int32_t* __restrict__ icount,
                                                      const int64_t* __restrict__ voff,
-                                                     const int64_t* __restrict__ ioff, int64_t NI, float param,
-                                                     float4* __restrict__ vout, int32_t* __restrict__ iout,
-                                                     float4* __restrict__ cout) {
+                                                     const int64_t* __restrict__ ioff, int64_t vcap, int64_t icap,
+                                                     float param, float4* __restrict__ vout,
+                                                     int32_t* __restrict__ iout, float4* __restrict__ cout) {
   const int64_t g = int64_t(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
   if (g * 8 >= n) return;
+  // totals (exclusive scans end with them): nothing is written past the
+  // caller's capacities; the caller reports the shortfall
+  const int64_t NV = voff[n];
+  const int64_t NI = ROUND ? ioff[n] : 0;
+  if (NV > vcap || NI > icap || NV > INT32_MAX) return;
   ColorQueue q{int(threadIdx.x >> 6), 0u, 0u};
   SeedLane L;
   L.lane = threadIdx.x & 63;
@@ -262,10 +267,17 @@ size_t finish_ws_bytes(int64_t n) {
   return size_t(n + 1) * 2 * sizeof(int64_t) + size_t(n) * 2 * sizeof(int32_t) + scan_workspace_bytes(n) + 4096;
 }
 
+__global__ void k_round_totals(const int64_t* __restrict__ voff, const int64_t* __restrict__ ioff, int64_t n,
+                               int64_t* __restrict__ totals) {
+  if (threadIdx.x == 0) {
+    totals[0] = voff[n];
+    totals[1] = ioff[n];
+  }
+}
+
 pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_t region, const int32_t* d_counts,
                         const int32_t* d_pairs, int mode, float param, float* d_vertices, size_t vertex_capacity,
-                        size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
-                        float* d_colors, void* ws) {
+                        int32_t* d_indices, size_t index_capacity, float* d_colors, void* ws, int64_t* totals) {
   if (region > kMaxRegion) return fail(ctx, PA_E_INVALID, "round: at most 2^24 slots per seed");
   char* w = static_cast<char*>(ws);
   auto* voff = reinterpret_cast<int64_t*>(w);
@@ -280,22 +292,15 @@ pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_
                      slots, n, region, d_counts, d_pairs, icount, cross);
   PA_TRY(scan_i32(ctx, d_counts, n, voff, sws));
   PA_TRY(scan_i32(ctx, icount, n, ioff, sws));
+  hipLaunchKernelGGL(k_round_totals, dim3(1), dim3(64), 0, ctx->stream, voff, ioff, n, totals);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
-  int64_t tot[2] = {0, 0};
-  PA_HIP(ctx, hipMemcpyAsync(&tot[0], voff + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  PA_HIP(ctx, hipMemcpyAsync(&tot[1], ioff + n, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *n_vertices = size_t(tot[0]);
-  *n_indices = size_t(tot[1]);
-  if (size_t(tot[0]) > vertex_capacity) return fail(ctx, PA_E_CAPACITY, "round: vertex capacity too small");
-  if (size_t(tot[1]) > index_capacity) return fail(ctx, PA_E_CAPACITY, "round: index capacity too small");
-  if (tot[0] > INT32_MAX) return fail(ctx, PA_E_CAPACITY, "round: indices are int32 (at most 2^31-1 vertices)");
   const int64_t groups = (n + 7) / 8;
   launch_begin(ctx, "round_finish");
   if (!with_color_mode(mode, [&](auto M) {
         hipLaunchKernelGGL((k_group8<true, decltype(M)::value>), dim3(unsigned((groups + 3) / 4)), dim3(kThreads), 0,
-                           ctx->stream, slots, n, region, d_counts, cross, icount, voff, ioff, tot[1], param,
+                           ctx->stream, slots, n, region, d_counts, cross, icount, voff, ioff,
+                           int64_t(vertex_capacity), int64_t(index_capacity), param,
                            reinterpret_cast<float4*>(d_vertices), d_indices, reinterpret_cast<float4*>(d_colors));
       }))
     return fail(ctx, PA_E_INVALID, "unknown color mode");
@@ -334,8 +339,8 @@ extern "C" pa_status pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d
   launch_begin(ctx, "prune_counted");
   hipLaunchKernelGGL((k_group8<false, 0>), dim3(unsigned((groups + 3) / 4)), dim3(kThreads), 0, ctx->stream,
                      reinterpret_cast<const float4*>(d_slots), int64_t(n), int64_t(region_slots), d_vertex_counts,
-                     nullptr, nullptr, offs, nullptr, int64_t(0), 0.0f, reinterpret_cast<float4*>(d_out),
-                     nullptr, nullptr);
+                     nullptr, nullptr, offs, nullptr, int64_t(capacity), int64_t(0), 0.0f,
+                     reinterpret_cast<float4*>(d_out), nullptr, nullptr);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;

@@ -16,10 +16,20 @@ pa_status group_handoffs(pa_gpu_ctx* ctx, const int32_t* d_code, const pa_partic
                          pa_handoff* handoff, void* ws);
 
 namespace {
+// Validation flags of the hybrid engine (bit 0: field index outside -1..5,
+// bit 1: block not loaded, bit 2: more iterations than region_slots, bit 3:
+// negative iterations).
+pa_status union_flags(pa_gpu_ctx* ctx, unsigned bad) {
+  if (bad & 1u) return fail(ctx, PA_E_INVALID, "vector_field_index outside -1..5");
+  if (bad & 8u) return fail(ctx, PA_E_INVALID, "negative remaining_iterations");
+  if (bad & 2u) return fail(ctx, PA_E_STATE, "a particle names a block that is not loaded");
+  if (bad & 4u) return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
+  return PA_OK;
+}
 pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_particles, size_t n,
                            size_t region_slots, pa_slot_order order, float* d_vertices, int32_t* d_vertex_counts,
                            pa_particle* d_continuation, size_t* n_continuation, float* d_endpoints,
-                           int32_t* d_seed_steps, uint64_t* total_steps, int32_t* pairs);
+                           int32_t* d_seed_steps, uint64_t* total_steps, int32_t* pairs, bool deferred);
 }  // namespace
 
 namespace {
@@ -744,7 +754,7 @@ pa_status pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* t, const pa_particle* 
                                            uint64_t* total_steps) {
   if (!t) return PA_E_INVALID;
   return trace_union_impl(t, t->engine, d_particles, n, region_slots, order, d_vertices, d_vertex_counts,
-                          d_continuation, n_continuation, d_endpoints, d_seed_steps, total_steps, nullptr);
+                          d_continuation, n_continuation, d_endpoints, d_seed_steps, total_steps, nullptr, false);
 }
 
 pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n,
@@ -772,12 +782,30 @@ pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d
   const size_t pair_bytes = (n * sizeof(int32_t) + 255) / 256 * 256;
   PA_TRY(scratch(ctx, pair_bytes + finish_ws_bytes(int64_t(n)), &ws));
   auto* pairs = static_cast<int32_t*>(ws);
+  // trace and finish are queued back to back; one read-back at the end
+  // carries the step count, the continuation count, the validation flags
+  // and the two totals
   PA_TRY(trace_union_impl(t, PA_ENGINE_HYBRID, d_particles, n, region_slots, PA_SLOTS_GROUP8, d_slots,
                           d_vertex_counts, d_continuation, n_continuation, d_endpoints, d_seed_steps, total_steps,
-                          pairs));
-  return finish_group8(ctx, d_slots, int64_t(n), int64_t(region_slots), d_vertex_counts, pairs, int(mode), parameter,
-                       d_vertices, vertex_capacity, n_vertices, d_indices, index_capacity, n_indices, d_colors,
-                       static_cast<char*>(ws) + pair_bytes);
+                          pairs, true));
+  char* cbase = static_cast<char*>(ctx->counters);
+  auto* totals = reinterpret_cast<int64_t*>(cbase + 1056);
+  PA_TRY(finish_group8(ctx, d_slots, int64_t(n), int64_t(region_slots), d_vertex_counts, pairs, int(mode), parameter,
+                       d_vertices, vertex_capacity, d_indices, index_capacity, d_colors,
+                       static_cast<char*>(ws) + pair_bytes, totals));
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+  PA_HIP(ctx, hipMemcpyAsync(h, cbase + 1024, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  PA_TRY(union_flags(ctx, unsigned(h[1] >> 32)));
+  if (total_steps) *total_steps = h[0];
+  *n_continuation = size_t(unsigned(h[1] & 0xffffffffu));
+  const int64_t tv = int64_t(h[4]), ti = int64_t(h[5]);
+  *n_vertices = size_t(tv);
+  *n_indices = size_t(ti);
+  if (size_t(tv) > vertex_capacity) return fail(ctx, PA_E_CAPACITY, "round: vertex capacity too small");
+  if (size_t(ti) > index_capacity) return fail(ctx, PA_E_CAPACITY, "round: index capacity too small");
+  if (tv > INT32_MAX) return fail(ctx, PA_E_CAPACITY, "round: indices are int32 (at most 2^31-1 vertices)");
+  return PA_OK;
 }
 
 }  // extern "C"
@@ -787,7 +815,7 @@ namespace {
 pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_particles, size_t n,
                            size_t region_slots, pa_slot_order order, float* d_vertices, int32_t* d_vertex_counts,
                            pa_particle* d_continuation, size_t* n_continuation, float* d_endpoints,
-                           int32_t* d_seed_steps, uint64_t* total_steps, int32_t* pairs) {
+                           int32_t* d_seed_steps, uint64_t* total_steps, int32_t* pairs, bool deferred) {
   pa_gpu_ctx* ctx = t->ctx;
   if (n_continuation) *n_continuation = 0;
   if (n == 0) {
@@ -798,11 +826,17 @@ pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_pa
   PA_TRY(check_integrator(ctx, t->integrator));
   TraceConfig cfg;
   PA_TRY(make_config(t, &cfg));
-  int maxr = 0;
-  PA_TRY(scan_particles(ctx, cfg, d_particles, n, &maxr));
-  if (maxr < 0) return fail(ctx, PA_E_INVALID, "negative remaining_iterations");
-  if (region_slots < size_t(maxr < 1 ? 1 : maxr))
-    return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
+  // The hybrid engine checks the particles itself (field index, loaded
+  // block, iterations against region_slots) and reports them with the
+  // counters; the others are checked here first.
+  if (engine != PA_ENGINE_HYBRID) {
+    int maxr = 0;
+    PA_TRY(scan_particles(ctx, cfg, d_particles, n, &maxr));
+    if (maxr < 0) return fail(ctx, PA_E_INVALID, "negative remaining_iterations");
+    if (region_slots < size_t(maxr < 1 ? 1 : maxr))
+      return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
+  }
+  if (region_slots < 1) return fail(ctx, PA_E_CAPACITY, "region_slots must be at least 1");
   if (order != PA_SLOTS_SEED_MAJOR && order != PA_SLOTS_STEP_MAJOR && order != PA_SLOTS_CHUNKED &&
       order != PA_SLOTS_GROUP8)
     return fail(ctx, PA_E_INVALID, "unknown slot order");
@@ -859,9 +893,11 @@ pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_pa
   launch_end(ctx);
   }
   PA_HIP(ctx, hipGetLastError());
+  if (deferred) return PA_OK;
   unsigned long long h[2] = {0, 0};
   PA_HIP(ctx, hipMemcpyAsync(h, steps, 16, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (engine == PA_ENGINE_HYBRID) PA_TRY(union_flags(ctx, unsigned(h[1] >> 32)));
   if (total_steps) *total_steps = h[0];
   *n_continuation = size_t(unsigned(h[1] & 0xffffffffu));
   return PA_OK;

@@ -436,7 +436,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
                                                              unsigned long long* __restrict__ steps_out, int rule,
                                                              float4* __restrict__ endpoints,
                                                              int32_t* __restrict__ seed_steps,
-                                                             int32_t* __restrict__ pairs_out) {
+                                                             int32_t* __restrict__ pairs_out,
+                                                             unsigned int* __restrict__ bad) {
   __shared__ QEntry tab[7][4];
   __shared__ QBlock blk[7];
   __shared__ float4 x_last[kQuadBlock / 64][kQuadSwitch];
@@ -460,6 +461,18 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     st.R = p.remaining_iterations;
     st.b = p.vector_field_index;
     st.orank = p.original_rank;
+    // A particle this call cannot trace is kept as its seed alone and
+    // flagged: field index outside -1..5 (1), its block not loaded (2),
+    // more iterations than slots (4), negative iterations (8).
+    const unsigned why = (st.b < -1 || st.b > 5) ? 1u
+                         : !cfg.has_field[st.b + 1] ? 2u
+                         : int64_t(st.R) > region ? 4u
+                         : st.R < 0 ? 8u : 0u;
+    if (why) {
+      atomicOr(bad, why);
+      st.b = -1;
+      st.R = 1;
+    }
     f = cfg.field[st.b + 1];
   }
 
@@ -607,7 +620,8 @@ pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode,
   dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
     hipLaunchKernelGGL((k_hybrid_union<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
                        dim3(unsigned(blocks)), dim3(kQuadBlock), 0, ctx->stream, cfg, l.particles, l.n, l.verts,
-                       l.region, l.counts, l.cont, ncont, steps, l.stop_rule, l.endpoints, l.seed_steps, pairs);
+                       l.region, l.counts, l.cont, ncont, steps, l.stop_rule, l.endpoints, l.seed_steps, pairs,
+                       ncont + 1);
   }, cfg.field[0].layout);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;

@@ -261,6 +261,39 @@ def test_union_round_fused(ctx, orc, storage, layout, mode):
     assert np.array_equal(idx.download(np.int32, ni), oidx)
 
 
+def test_union_round_rejects_bad_particles(ctx, orc):
+    """The fused round checks the particles on the device (the hybrid engine
+    has no host pre-pass): a field index outside -1..5, a block that is not
+    loaded, more iterations than region_slots and negative iterations each
+    fail with their own status, and the next good call is unaffected."""
+    d, o, _ = single_block(ctx, orc, 24)
+    t = pa.particle_tracer(ctx)
+    t.set_local_vector_field(d)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.02)
+    t.set_engine(pa.ENGINE_HYBRID)
+    seeds = orc.seed_generate((0, 0, 0), o.size, [f32(4) * o.spacing[0]] * 3, 40, 0)
+    region = 48
+    bufs = t.union_buffers(len(seeds), region, endpoints=True)
+
+    def run(p):
+        return t.trace_union_round(pa.Particles.from_host(ctx, p), region, buffers=bufs)
+
+    good = run(seeds)
+    want_v = good["vertices"].download(np.float32, 4 * good["n_vertices"]).tobytes()
+    cases = [("vector_field_index", 7, pa.E_INVALID), ("vector_field_index", 2, pa.E_STATE),
+             ("remaining_iterations", region + 1, pa.E_CAPACITY), ("remaining_iterations", -3, pa.E_INVALID)]
+    for key, value, status in cases:
+        bad = seeds.copy()
+        bad[key][len(bad) // 2] = value
+        with pytest.raises(pa.PaError) as e:
+            run(bad)
+        assert e.value.status == status, (key, value)
+    again = run(seeds)
+    assert again["n_vertices"] == good["n_vertices"] and again["n_indices"] == good["n_indices"]
+    assert again["vertices"].download(np.float32, 4 * again["n_vertices"]).tobytes() == want_v
+
+
 def test_prune_index_color_parity(ctx, orc):
     d, o, _ = single_block(ctx, orc, 48)
     h = o.spacing[0]
