@@ -48,12 +48,13 @@ static void drain(pa_gpu_ctx* ctx, bool wait) {
       keep.push_back(t);
       continue;
     }
-    hipEventSynchronize(t.b);
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, t.a, t.b);
-    auto& slot = ctx->kernel_ms[t.name];
-    slot.first += double(ms);
-    slot.second += 1;
+    // a pair whose events did not complete is not counted
+    if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      auto& slot = ctx->kernel_ms[t.name];
+      slot.first += double(ms);
+      slot.second += 1;
+    }
     ctx->event_pool.push_back(t.a);
     ctx->event_pool.push_back(t.b);
   }
@@ -67,13 +68,22 @@ void launch_begin(pa_gpu_ctx* ctx, const char* name) {
   t.b = take_event(ctx);
   t.name = name;
   if (!t.a || !t.b) return;
-  hipEventRecord(t.a, ctx->stream);
+  if (hipEventRecord(t.a, ctx->stream) != hipSuccess) {
+    ctx->event_pool.push_back(t.a);
+    ctx->event_pool.push_back(t.b);
+    return;
+  }
   ctx->pending.push_back(t);
 }
 
 void launch_end(pa_gpu_ctx* ctx) {
   if (!ctx->timing || ctx->pending.empty()) return;
-  hipEventRecord(ctx->pending.back().b, ctx->stream);
+  if (hipEventRecord(ctx->pending.back().b, ctx->stream) != hipSuccess) {
+    ctx->event_pool.push_back(ctx->pending.back().a);
+    ctx->event_pool.push_back(ctx->pending.back().b);
+    ctx->pending.pop_back();
+    return;
+  }
   if (ctx->pending.size() > 256) drain(ctx, true);
 }
 
@@ -97,13 +107,13 @@ pa_status pa_gpu_ctx_create(int device, pa_gpu_ctx** out) {
     return PA_E_DEVICE;
   }
   if (hipMalloc(&ctx->counters, 4096) != hipSuccess) {
-    hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PA_E_NOMEM;
   }
   if (hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
-    hipFree(ctx->counters);
-    hipStreamDestroy(ctx->stream);
+    (void)hipFree(ctx->counters);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PA_E_NOMEM;
   }
@@ -113,14 +123,15 @@ pa_status pa_gpu_ctx_create(int device, pa_gpu_ctx** out) {
 
 pa_status pa_gpu_ctx_destroy(pa_gpu_ctx* ctx) {
   if (!ctx) return PA_E_INVALID;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   drain(ctx, true);
-  for (auto e : ctx->event_pool) hipEventDestroy(e);
-  if (ctx->scratch) hipFree(ctx->scratch);
-  if (ctx->counters) hipFree(ctx->counters);
-  if (ctx->pinned) hipHostFree(ctx->pinned);
-  hipStreamDestroy(ctx->stream);
+  // teardown: failures here have nowhere to go
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->counters) (void)hipFree(ctx->counters);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PA_OK;
 }

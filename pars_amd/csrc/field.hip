@@ -222,8 +222,8 @@ pa_status pa_gpu_field_create(pa_gpu_ctx* ctx, const float* h_xyz, const int32_t
   hipError_t e = hipMemcpyAsync(staging, h_xyz, n * 12, hipMemcpyHostToDevice, ctx->stream);
   pa_status s = e == hipSuccess ? pa_gpu_field_create_device(ctx, staging, dims, offset, size, spacing, storage, out)
                                 : check_hip(ctx, e, "hipMemcpy(field)");
-  hipStreamSynchronize(ctx->stream);
-  hipFree(staging);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(staging);
   return s;
 }
 
@@ -262,7 +262,7 @@ pa_status pa_gpu_field_read(pa_gpu_ctx* ctx, const pa_gpu_field* field, float* h
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(h_xyz, staging, size_t(n) * 12, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  hipFree(staging);
+  (void)hipFree(staging);
   if (e != hipSuccess) return check_hip(ctx, e, "field_read");
   return PA_OK;
 }

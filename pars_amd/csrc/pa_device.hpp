@@ -247,7 +247,7 @@ template <int STOP>
 __device__ __forceinline__ bool stops(const float v[3], float eps2, int rule = STOP) {
   if constexpr (STOP == STOP_RUNTIME) {
     const float n2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
-    const bool zero = (fabsf(v[0]) <= 1e-5f) & (fabsf(v[1]) <= 1e-5f) & (fabsf(v[2]) <= 1e-5f);
+    const bool zero = (int(fabsf(v[0]) <= 1e-5f) & int(fabsf(v[1]) <= 1e-5f) & int(fabsf(v[2]) <= 1e-5f)) != 0;
     return rule == PA_STOP_NORM ? n2 < eps2 : zero;
   } else if constexpr (STOP == PA_STOP_NORM) {
     const float n2 = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];

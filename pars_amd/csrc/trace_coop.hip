@@ -421,7 +421,7 @@ __global__ void k_zero2(unsigned long long* c) {
 int coop_waves(pa_gpu_ctx* ctx, int64_t n, int requested) {
   if (requested > 0) return requested;
   int cus = 256;
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
   int64_t w = (n + kWave - 1) / kWave;
   const int64_t cap = int64_t(cus) * 12;
   if (w > cap) w = cap;
