@@ -159,10 +159,12 @@ pa_status scan_i32(pa_gpu_ctx* ctx, const int32_t* d_in, int64_t n, int64_t* d_o
 // Asynchronous: the totals land in totals[0] (vertices) and totals[1]
 // (indices) on the device; nothing is written when they exceed the
 // capacities (or 2^31-1 vertices), which the caller checks after its one
-// synchronisation.
+// synchronisation; a finish wave that gives up a bounded wait sets bit 16
+// of *flags.
 pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_t region, const int32_t* d_counts,
                         const int32_t* d_pairs, int mode, float param, float* d_vertices, size_t vertex_capacity,
-                        int32_t* d_indices, size_t index_capacity, float* d_colors, void* ws, int64_t* totals);
+                        int32_t* d_indices, size_t index_capacity, float* d_colors, void* ws, int64_t* totals,
+                        unsigned* flags);
 size_t finish_ws_bytes(int64_t n);
 // The colour stage's kernel (one lane per index).
 pa_status color_launch(pa_gpu_ctx* ctx, const float4* d_vertices, const int32_t* d_indices, int64_t n_indices,

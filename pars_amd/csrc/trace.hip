@@ -24,6 +24,7 @@ pa_status union_flags(pa_gpu_ctx* ctx, unsigned bad) {
   if (bad & 8u) return fail(ctx, PA_E_INVALID, "negative remaining_iterations");
   if (bad & 2u) return fail(ctx, PA_E_STATE, "a particle names a block that is not loaded");
   if (bad & 4u) return fail(ctx, PA_E_CAPACITY, "region_slots must be at least the maximum remaining_iterations");
+  if (bad & (1u << 16)) return fail(ctx, PA_E_DEVICE, "round: a finish wave stopped waiting (outputs incomplete)");
   return PA_OK;
 }
 pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_particles, size_t n,
@@ -792,7 +793,7 @@ pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d
   auto* totals = reinterpret_cast<int64_t*>(cbase + 1056);
   PA_TRY(finish_group8(ctx, d_slots, int64_t(n), int64_t(region_slots), d_vertex_counts, pairs, int(mode), parameter,
                        d_vertices, vertex_capacity, d_indices, index_capacity, d_colors,
-                       static_cast<char*>(ws) + pair_bytes, totals));
+                       static_cast<char*>(ws) + pair_bytes, totals, reinterpret_cast<unsigned*>(cbase + 1036)));
   unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
   PA_HIP(ctx, hipMemcpyAsync(h, cbase + 1024, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
