@@ -338,7 +338,11 @@ pa_status   pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* tracer, const pa_par
  * segment colours are left untouched).  Replaces the stage sequence
  * particle_tracer::trace (pars/source/pipeline.cpp:186-189) -> prune
  * (:213-220) -> index_generator (:224-234) -> color_generator (:238-260)
- * for one rank's share traced over its 7 preloaded blocks. */
+ * for one rank's share traced over its 7 preloaded blocks.  The particles
+ * are checked on the device: a field index outside -1..5 (PA_E_INVALID), a
+ * block that is not loaded (PA_E_STATE), remaining_iterations above
+ * region_slots (PA_E_CAPACITY) or negative (PA_E_INVALID) fail the call;
+ * region_slots is at most 2^24.  The call reads back once, at its end. */
 pa_status   pa_gpu_tracer_trace_union_round(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
                                             size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
                                             pa_particle* d_continuation, size_t* n_continuation,

@@ -109,6 +109,9 @@ class vector_field {
   vector_field(const vector_field&) = delete;
   vector_field& operator=(const vector_field&) = delete;
   const pa_gpu_field* get() const { return f_; }
+  // Extension: also keep the 8-corner cell records (PA_LAYOUT_CELLS), one
+  // cache line per trilinear gather.
+  void set_layout(pa_layout layout) { c_->check(pa_gpu_field_set_layout(c_->get(), f_, layout), "set_layout"); }
 
   vector3 offset, size, spacing;
   std::array<int32_t, 3> dims;
@@ -160,6 +163,59 @@ class particle_tracer {
   }
   void set_integrator(pa_integrator i) { c_->check(pa_gpu_tracer_set_integrator(t_, i), "set_integrator"); }
   void set_step_size(scalar s) { c_->check(pa_gpu_tracer_set_step_size(t_, s), "set_step_size"); }
+  // Extensions (defaults are the reference's behaviour).
+  void set_stop_rule(pa_stop_rule r, float eps = 0.0f) {
+    c_->check(pa_gpu_tracer_set_stop_rule(t_, r, eps), "set_stop_rule");
+  }
+  void set_interpolation(pa_interp i) { c_->check(pa_gpu_tracer_set_interpolation(t_, i), "set_interpolation"); }
+  void set_engine(pa_engine e, int32_t waves = 0) { c_->check(pa_gpu_tracer_set_engine(t_, e, waves), "set_engine"); }
+  void set_neighbor_face_masks(const std::array<int32_t, 6>& masks) {
+    c_->check(pa_gpu_tracer_set_neighbor_face_masks(t_, masks.data()), "set_neighbor_face_masks");
+  }
+  pa_gpu_tracer* get() const { return t_; }
+
+  // Extension: one rank's whole round over its 7 preloaded blocks, fused on
+  // the device (union trace -> prune -> index -> colour,
+  // pars/source/pipeline.cpp:186-260); the curves are those the stage chain
+  // produces on the traced slots.  Particles whose segment would not fit in
+  // region_slots come back in `continuation` for the next round.
+  struct union_round {
+    integral_curves curves;
+    std::vector<particle> continuation;
+    std::vector<int32_t> counts;    // vertices per seed (in seed order)
+    std::vector<vector4> endpoints;  // last position per seed
+    uint64_t total_steps = 0;
+  };
+  union_round trace_union_round(const std::vector<particle>& seeds, size_t region_slots,
+                                pa_color_mode mode = PA_COLOR_HSV_CONSTANT_S, scalar parameter = 0.75f) {
+    union_round out;
+    const size_t n = seeds.size();
+    if (n == 0) return out;
+    device_vector<particle> d(*c_, seeds);
+    const size_t groups = (n + 7) / 8;
+    device_vector<vector4> slots(*c_, groups * 8 * region_slots);
+    device_vector<int32_t> counts(*c_, n);
+    device_vector<particle> cont(*c_, n);
+    device_vector<vector4> ends(*c_, n);
+    const size_t cap = n * region_slots;
+    device_vector<vector4> vertices(*c_, cap);
+    device_vector<int32_t> indices(*c_, cap);
+    device_vector<vector4> colors(*c_, cap);
+    size_t ncont = 0, nv = 0, ni = 0;
+    c_->check(pa_gpu_tracer_trace_union_round(t_, d.data(), n, region_slots, reinterpret_cast<float*>(slots.data()),
+                                              counts.data(), cont.data(), &ncont, mode, parameter,
+                                              reinterpret_cast<float*>(vertices.data()), cap, &nv, indices.data(),
+                                              cap, &ni, reinterpret_cast<float*>(colors.data()),
+                                              reinterpret_cast<float*>(ends.data()), nullptr, &out.total_steps),
+              "trace_union_round");
+    out.curves.vertices = vertices.to_host(nv);
+    out.curves.indices = indices.to_host(ni);
+    out.curves.colors = colors.to_host(nv);
+    out.continuation = cont.to_host(ncont);
+    out.counts = counts.to_host(n);
+    out.endpoints = ends.to_host(n);
+    return out;
+  }
 
   // compute_round_info + allocate + initialize + trace + prune of one round
   // (particle_tracer.cpp:210-337, 402-409): returns the pruned curves of the
@@ -216,6 +272,52 @@ class particle_tracer {
  private:
   context* c_;
   pa_gpu_tracer* t_ = nullptr;
+};
+
+// Extension: particle pool over up to 16 resident fields (batched seed sets,
+// the step engine of continuous-batching serving; pars serves one request at
+// a time, pars/source/service.cpp:20-69).  Each particle ends where a
+// one-block trace of it on its field (vector_field_index) ends.
+class particle_pool {
+ public:
+  particle_pool(context& c, size_t capacity, size_t requests = 0)
+      : c_(&c), cap_(capacity), position_(c, capacity), iteration_(c, capacity), remaining_(c, capacity),
+        steps_(c, capacity), status_(c, capacity), field_(c, capacity), request_(c, capacity),
+        request_done_(c, std::vector<int32_t>(requests ? requests : 1, 0)) {
+    pool_ = pa_pool{reinterpret_cast<float*>(position_.data()), iteration_.data(), remaining_.data(), steps_.data(),
+                    status_.data(), field_.data(), request_.data(), request_done_.data()};
+  }
+  // Load particles (vector_field_index selects the field, original_rank the
+  // request).
+  void init(const std::vector<particle>& particles, int32_t n_fields) {
+    if (particles.size() > cap_) throw error(PA_E_CAPACITY, "particle_pool: more particles than capacity");
+    n_ = particles.size();
+    if (n_ == 0) return;
+    device_vector<particle> d(*c_, particles);
+    c_->check(pa_gpu_pool_init(c_->get(), d.data(), n_, n_fields, &pool_), "pool_init");
+  }
+  // Advance by at most max_iterations (0: to the end); returns the steps taken.
+  uint64_t run(particle_tracer& tracer, const std::vector<const vector_field*>& fields, int32_t max_iterations = 0,
+               int32_t waves_per_cu = 0) {
+    std::vector<const pa_gpu_field*> f;
+    for (auto* v : fields) f.push_back(v->get());
+    uint64_t steps = 0;
+    c_->check(pa_gpu_pool_run(tracer.get(), f.data(), int32_t(f.size()), &pool_, n_, max_iterations, waves_per_cu,
+                              &steps),
+              "pool_run");
+    return steps;
+  }
+  std::vector<vector4> positions() const { return position_.to_host(n_); }
+  std::vector<int32_t> steps() const { return steps_.to_host(n_); }
+  std::vector<int32_t> status() const { return status_.to_host(n_); }
+  std::vector<int32_t> request_done() const { return request_done_.to_host(request_done_.size()); }
+
+ private:
+  context* c_;
+  size_t cap_, n_ = 0;
+  device_vector<vector4> position_;
+  device_vector<int32_t> iteration_, remaining_, steps_, status_, field_, request_, request_done_;
+  pa_pool pool_{};
 };
 
 // pa::index_generator (index_generator.hpp:10-14).

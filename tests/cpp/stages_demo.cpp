@@ -1,8 +1,9 @@
 // Drives one streamline pass through the C++ stage mirror (pa_amd/stages.hpp)
 // the way pars/source/pipeline.cpp:107-260 drives pa::: seeds over the own
-// block, one trace round, prune, index, colour.  Used by
-// tests/test_gpu_cpp_mirror.py, which compares the files it writes against
-// the oracle.
+// block, one trace round, prune, index, colour; then the same round fused on
+// the device and the same seeds through the particle pool in bounded passes.
+// Used by tests/test_gpu_cpp_mirror.py, which compares the files it writes
+// against the oracle and against each other.
 //
 // usage: stages_demo <field.f32> <n> <R> <dt> <out_prefix>
 //   field.f32: n^3 x 3 float32 (C order, z fastest), spacing 2pi/(n-1)
@@ -58,8 +59,30 @@ int main(int argc, char** argv) {
     dump(out + ".vertices", c.vertices);
     dump(out + ".indices", c.indices);
     dump(out + ".colors", c.colors);
-    std::printf("stages_demo: %zu seeds, %zu vertices, %zu indices\n", seeds.size(), c.vertices.size(),
-                c.indices.size());
+    // the same round fused on the device (extension), and the particle pool
+    pa_amd::particle_tracer fused(ctx);
+    fused.set_local_vector_field(&field);
+    fused.set_integrator(PA_INTEGRATOR_RUNGE_KUTTA_4);
+    fused.set_step_size(dt);
+    auto round = fused.trace_union_round(seeds, size_t(R), PA_COLOR_HSV_CONSTANT_S, 0.75f);
+    dump(out + ".round_vertices", round.curves.vertices);
+    dump(out + ".round_indices", round.curves.indices);
+    dump(out + ".round_colors", round.curves.colors);
+    dump(out + ".round_endpoints", round.endpoints);
+    std::vector<pa_amd::particle> pool_seeds = seeds;
+    for (auto& p : pool_seeds) p.vector_field_index = 0;  // field 0 of the pool
+    pa_amd::particle_pool pool(ctx, pool_seeds.size());
+    pool.init(pool_seeds, 1);
+    uint64_t pool_steps = 0;
+    while (true) {  // bounded passes of 40 iterations, as a serving loop runs them
+      const uint64_t s = pool.run(fused, {&field}, 40);
+      pool_steps += s;
+      if (s == 0) break;
+    }
+    dump(out + ".pool_positions", pool.positions());
+    std::printf("stages_demo: %zu seeds, %zu vertices, %zu indices; round %zu vertices, %llu steps; pool %llu steps\n",
+                seeds.size(), c.vertices.size(), c.indices.size(), round.curves.vertices.size(),
+                static_cast<unsigned long long>(round.total_steps), static_cast<unsigned long long>(pool_steps));
   } catch (const pa_amd::error& e) {
     std::fprintf(stderr, "pa_amd error %d: %s\n", int(e.status), e.what());
     return 1;

@@ -37,4 +37,22 @@ def test_cpp_mirror_pipeline(orc, tmp_path):
     idx = orc.index_generate(v)
     assert np.array_equal(np.fromfile(out + ".indices", np.int32), idx)
     col = orc.color_generate(v, idx, 2, 0.75)
-    np.testing.assert_allclose(np.fromfile(out + ".colors", np.float32).reshape(-1, 4), col, atol=1e-6)
+    got_c = np.fromfile(out + ".colors", np.float32).reshape(-1, 4)
+    np.testing.assert_allclose(got_c, col, atol=1e-6)
+    # the fused round on one block produces the stage chain's curves bit for bit
+    assert np.fromfile(out + ".round_vertices", np.float32).tobytes() == got_v.tobytes()
+    assert np.array_equal(np.fromfile(out + ".round_indices", np.int32), idx)
+    # colours are compared where the colour stage writes them (the others
+    # are undefined in the reference: colors.resize leaves them uninitialised)
+    rc = np.fromfile(out + ".round_colors", np.float32).reshape(-1, 4)
+    written = np.zeros(len(v), bool)
+    written[idx] = True
+    ends = np.ones(len(idx), bool)
+    ends[:-1] = idx[1:] != idx[:-1] + 1
+    ends[-1] = False
+    written[idx[ends] + 1] = True
+    assert rc[written].tobytes() == got_c[written].tobytes()
+    # the pool ends every seed where the round ends it
+    ends_round = np.fromfile(out + ".round_endpoints", np.float32).reshape(-1, 4)
+    pool_pos = np.fromfile(out + ".pool_positions", np.float32).reshape(-1, 4)
+    assert pool_pos.tobytes() == ends_round.tobytes()
