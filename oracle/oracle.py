@@ -305,6 +305,8 @@ def endpoints(particles, local, neighbors=None, neighbor_rank=None, face_masks=N
 def endpoints_f64(positions, remaining_iterations, local, neighbors=None, neighbor_rank=None, face_masks=None,
                   dt=0.01, integrator=RK4, union=False, stop_rule=STOP_ISZERO, stop_eps=0.0):
     pos = np.ascontiguousarray(positions, np.float64)
+    if pos.ndim != 2 or pos.shape[1] != 4:
+        raise ValueError("endpoints_f64: positions are (n, 4) x, y, z, w")
     out = np.zeros((len(pos), 4), np.float64)
     steps = np.zeros(len(pos), np.int32)
     cfg = _config(local, neighbors, neighbor_rank, face_masks, dt, integrator, stop_rule, stop_eps, f64=True)
