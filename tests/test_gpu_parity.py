@@ -294,6 +294,36 @@ def test_union_round_rejects_bad_particles(ctx, orc):
     assert again["vertices"].download(np.float32, 4 * again["n_vertices"]).tobytes() == want_v
 
 
+def test_union_round_capacity(ctx, orc):
+    """A fused round whose vertex or index capacity is too small fails with
+    PA_E_CAPACITY, reports the needed sizes and writes nothing; the same
+    call with room succeeds."""
+    d, o, _ = single_block(ctx, orc, 24)
+    t = pa.particle_tracer(ctx)
+    t.set_local_vector_field(d)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.02)
+    seeds = orc.seed_generate((0, 0, 0), o.size, [f32(3) * o.spacing[0]] * 3, 40, 0)
+    n, region = len(seeds), 48
+    parts = pa.Particles.from_host(ctx, seeds)
+    b = t.union_buffers(n, region, endpoints=True)
+    good = t.trace_union_round(parts, region, buffers=b)
+    nv, ni = good["n_vertices"], good["n_indices"]
+    lib, C = pa.lib(), pa.C
+    sentinel = np.full(4 * nv, 7.25, np.float32)
+    for vcap, icap in ((nv - 1, nv), (nv, ni - 1)):
+        verts, idx, cols = ctx.upload(sentinel), ctx.alloc(4 * nv), ctx.alloc(16 * nv)
+        ncont, gv, gi, steps = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_uint64()
+        st = lib.pa_gpu_tracer_trace_union_round(
+            t._h, C.c_void_p(parts.ptr), C.c_size_t(n), C.c_size_t(region), C.c_void_p(b["vertices"].ptr),
+            C.c_void_p(b["counts"].ptr), C.c_void_p(b["cont"].ptr), C.byref(ncont), C.c_int(2), C.c_float(0.75),
+            C.c_void_p(verts.ptr), C.c_size_t(vcap), C.byref(gv), C.c_void_p(idx.ptr), C.c_size_t(icap), C.byref(gi),
+            C.c_void_p(cols.ptr), C.c_void_p(b["endpoints"].ptr), C.c_void_p(b["seed_steps"].ptr), C.byref(steps))
+        assert st == pa.E_CAPACITY
+        assert (gv.value, gi.value) == (nv, ni)
+        assert verts.download(np.float32, 4 * nv).tobytes() == sentinel.tobytes()
+
+
 def test_prune_index_color_parity(ctx, orc):
     d, o, _ = single_block(ctx, orc, 48)
     h = o.spacing[0]
