@@ -173,6 +173,65 @@ __device__ __forceinline__ void lerp_corners(const float* c, const float w[3], c
   }
 }
 
+// Cell records (PA_LAYOUT_CELLS): one record per cell holds x of c000 c001
+// c010 c011 c100 c101 c110 c111, then y, then z; FP32 records are 8 uint4
+// (6 used), 16-bit records 4 uint4 (3 used).
+template <int S>
+constexpr int rec_used() { return storage_of(S) == PA_STORAGE_FP32 ? 6 : 3; }
+template <int S>
+constexpr int rec_size() { return storage_of(S) == PA_STORAGE_FP32 ? 8 : 4; }
+
+// The record of the cell with subscripts m (fewer than 2^31 records,
+// checked when the layout is built).
+template <int S>
+__device__ __forceinline__ const uint4* cell_record(const FieldDesc& f, const int m[3]) {
+  const uint32_t ci = uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
+  return static_cast<const uint4*>(f.cells) + uint64_t(ci) * rec_size<S>();
+}
+
+// The 24 corner values of a record (its rec_used<S>() uint4 as loaded),
+// component-planar: k[comp * 8 + corner].
+template <int S>
+__device__ __forceinline__ void decode_record(const uint4* r, float k[24]) {
+  if constexpr (storage_of(S) == PA_STORAGE_FP32) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      k[4 * i + 0] = __uint_as_float(r[i].x);
+      k[4 * i + 1] = __uint_as_float(r[i].y);
+      k[4 * i + 2] = __uint_as_float(r[i].z);
+      k[4 * i + 3] = __uint_as_float(r[i].w);
+    }
+  } else {
+    constexpr int st = storage_of(S);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      k[8 * i + 0] = dec16(r[i].x, st); k[8 * i + 1] = dec16(r[i].x >> 16, st);
+      k[8 * i + 2] = dec16(r[i].y, st); k[8 * i + 3] = dec16(r[i].y >> 16, st);
+      k[8 * i + 4] = dec16(r[i].z, st); k[8 * i + 5] = dec16(r[i].z >> 16, st);
+      k[8 * i + 6] = dec16(r[i].w, st); k[8 * i + 7] = dec16(r[i].w >> 16, st);
+    }
+  }
+}
+
+// The lerp tree of vector_field.cpp:41-47 (z, then y, then x) over
+// component-planar corners k[comp * 8 + corner], corner order c000 c001
+// c010 c011 c100 c101 c110 c111.
+template <int INTERP>
+__device__ __forceinline__ void lerp_planar(const FieldDesc& f, const float d[3], const float q[3], const float k[24],
+                                            float v[3]) {
+  int m[3];
+  float w[3], u[3];
+  cell_weights<INTERP>(f, d, q, m, w, u);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float* c = k + 8 * i;
+    const float c00 = lerp1(c[0], c[1], w[2], u[2]), c01 = lerp1(c[2], c[3], w[2], u[2]);
+    const float c10 = lerp1(c[4], c[5], w[2], u[2]), c11 = lerp1(c[6], c[7], w[2], u[2]);
+    const float c0 = lerp1(c00, c01, w[1], u[1]), c1 = lerp1(c10, c11, w[1], u[1]);
+    v[i] = lerp1(c0, c1, w[0], u[0]);
+  }
+}
+
 // vector_field::interpolate (vector_field.cpp:22-48): m = floor(q), w =
 // fmod(d, h) / h, lerp (linear_interpolate.hpp:6-10) along z, y, x.  With
 // INTERP == PA_INTERP_CONSISTENT the weight is q - floor(q) (exact in FP32

@@ -22,7 +22,7 @@ namespace pa_amd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kUnroll = 4;  // 8-slot chunks per pass of k_group8
+constexpr int kUnroll = 4;  // 8-slot rows per register block of a packing wave
 
 // icount[p] = in-run pairs + the pair (last of p, first of p+1).
 __global__ __launch_bounds__(kThreads) void k_round_counts(const float4* __restrict__ slots, int64_t n, int64_t region,
@@ -79,7 +79,7 @@ __device__ __forceinline__ int64_t uniform(int64_t x) {
   return int64_t((uint64_t(hi) << 32) | lo);
 }
 
-// What one lane of k_group8 knows about its seed p (lane 8k + j of the wave
+// What one lane of a packing wave knows about its seed p (lane 8k + j of the wave
 // holds slot s0 + j of seed k of the group).
 struct SeedLane {
   int64_t O;         // first packed vertex of the seed

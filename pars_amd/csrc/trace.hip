@@ -385,6 +385,122 @@ __global__ __launch_bounds__(kBlock) void k_flow_map(const TraceConfig cfg, int6
   if ((threadIdx.x & 63) == 0 && steps) atomicAdd(steps_out, steps);
 }
 
+// Cell-record gathers shared by a wave.  Loaded lane by lane, one wave load
+// instruction touches 64 cache lines (one record per lane); here wave load i
+// covers chunks 64 i .. 64 i + 63 of the wave's records in order (lane l:
+// chunk (64 i + l) % U of record (64 i + l) / U), so it touches about 64 / U
+// lines, and goes straight to LDS (no staging registers); every lane then
+// reads its own record from LDS.  Lanes whose record is not needed load a
+// chunk of a needed one (no extra lines).
+template <int S>
+struct CoopRecords {
+  uint64_t addr[64];
+  uint4 rec[64][rec_used<S>()];
+};
+
+// After the call lane l's record is sh.rec[l]; the lane reads it before the
+// wave's next gather rewrites it (a wave executes in order).
+template <int S>
+__device__ __forceinline__ void coop_gather(CoopRecords<S>& sh, bool want, const uint4* mine, int lane) {
+  constexpr int U = rec_used<S>();
+  const uint64_t W = __ballot(want);
+  if (W == 0) return;
+  sh.addr[lane] = reinterpret_cast<uint64_t>(mine);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint64_t any = sh.addr[__builtin_ctzll(W)];
+  auto* dst = (__attribute__((address_space(3))) uint4*)(&sh.rec[0][0]);
+#pragma unroll
+  for (int i = 0; i < U; ++i) {
+    const int g = i * 64 + lane, src = g / U, c = g % U;
+    const uint64_t a = ((W >> src) & 1) ? sh.addr[src] : any;
+    __builtin_amdgcn_global_load_lds(reinterpret_cast<const __attribute__((address_space(1))) void*>(a + 16u * c),
+                                     dst + i * 64, 16, 0, 0);
+  }
+  // Loads into LDS count in vmcnt and no fence waits for them: wait here
+  // (vmcnt 0; expcnt and lgkmcnt left at their maxima) before any lane
+  // reads a record.
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// k_flow_map for cell records with one gather per step: the advection of
+// advect_one (particle_advector.cpp:61-141) as a wave-level loop, so the
+// wave can gather its records together (coop_gather).  Same arithmetic,
+// same results bit for bit.
+template <int S, int MODE, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_flow_map_cells(const TraceConfig cfg, int64_t n, int sd1, int sd2,
+                                                           float o0, float o1, float o2, float s0, float s1, float s2,
+                                                           int32_t R0, float* __restrict__ fm, int fd0, int fd1,
+                                                           int fd2, unsigned long long* __restrict__ steps_out,
+                                                           int stop_rule) {
+  __shared__ CoopRecords<S> sh[kBlock / 64];
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const FieldDesc& f = cfg.field[0];
+  unsigned long long steps = 0;
+  int x = 0, j = 0, k = 0;
+  float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int R = R0, it = 1;
+  int state = 3;  // advect_one's states: 1 inactive (written), 2 hand-off, 3 never stepped; 0 running
+  if (i < n) {
+    k = int(i % sd2);
+    j = int((i / sd2) % sd1);
+    x = int(i / (int64_t(sd1) * sd2));
+    pos = make_float4(o0 + s0 * float(x), o1 + s1 * float(j), o2 + s2 * float(k), 0.0f);
+    if (R > 1) state = 0;
+  }
+  for (;;) {
+    const bool run = state == 0;
+    if (__ballot(run) == 0) break;
+    float d[3], q[3];
+    bool inside = false;
+    const uint4* mine = nullptr;
+    if (run) {
+      inside = locate(f, pos.x, pos.y, pos.z, d, q);
+      if (inside) {
+        int m[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) m[a] = static_cast<int>(floorf(q[a]));
+        mine = cell_record<S>(f, m);
+      }
+    }
+    coop_gather<S>(sh[w], inside, mine, lane);
+    if (run) {
+      float4 next;
+      int a = 1;
+      if (inside) {
+        float kv[24], v[3];
+        decode_record<S>(sh[w].rec[lane], kv);
+        lerp_planar<INTERP>(f, d, q, kv, v);
+        a = finish_step<S, MODE, INTERP, STOP_RUNTIME>(cfg, f, pos, v, next, stop_rule);
+      }
+      if (a == 1) {  // :76-100
+        R -= it;
+        state = face_rule(f, pos, cfg.local_mask) < 0 ? 1 : 2;
+      } else if (a == 2) {  // :102-109
+        state = 1;
+      } else {
+        pos = next;
+        ++steps;
+        if (it + 1 == R) state = 1;  // :132-138
+        else ++it;
+      }
+    }
+  }
+  if (i < n && state == 1 && x < fd0 && j < fd1 && k < fd2) {
+    const int64_t at = (int64_t(x) * fd1 + j) * fd2 + k;
+    fm[3 * at] = pos.x;
+    fm[3 * at + 1] = pos.y;
+    fm[3 * at + 2] = pos.z;
+  }
+  steps = wave_sum_u64(steps);
+  if (lane == 0 && steps) atomicAdd(steps_out, steps);
+}
+
 // ------------------------------------------------------------ dispatching
 int mode_of(int32_t integrator) {
   if (integrator == PA_INTEGRATOR_EULER) return STEP_EULER;
@@ -976,10 +1092,17 @@ pa_status pa_gpu_flow_map_generate(pa_gpu_tracer* t, const int32_t block_size[3]
     const int mode = mode_of(t->integrator);
     launch_begin(ctx, "flow_map");
     dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
-      hipLaunchKernelGGL((k_flow_map<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
-                         dim3(grid_of(int64_t(ns))), dim3(kBlock), 0, ctx->stream, cfg, int64_t(ns), sd[1], sd[2],
-                         f.offset[0], f.offset[1], f.offset[2], spacing[0], spacing[1], spacing[2],
-                         int32_t(iterations), d_flow_map, dims_out[0], dims_out[1], dims_out[2], steps, t->stop_rule);
+      constexpr int SS = decltype(S)::value, MM = decltype(M)::value, II = decltype(I)::value;
+      if constexpr (layout_of(SS) == PA_LAYOUT_CELLS && MM != STEP_CLASSIC)
+        hipLaunchKernelGGL((k_flow_map_cells<SS, MM, II>), dim3(grid_of(int64_t(ns))), dim3(kBlock), 0, ctx->stream,
+                           cfg, int64_t(ns), sd[1], sd[2], f.offset[0], f.offset[1], f.offset[2], spacing[0],
+                           spacing[1], spacing[2], int32_t(iterations), d_flow_map, dims_out[0], dims_out[1],
+                           dims_out[2], steps, t->stop_rule);
+      else
+        hipLaunchKernelGGL((k_flow_map<SS, MM, II>), dim3(grid_of(int64_t(ns))), dim3(kBlock), 0, ctx->stream, cfg,
+                           int64_t(ns), sd[1], sd[2], f.offset[0], f.offset[1], f.offset[2], spacing[0], spacing[1],
+                           spacing[2], int32_t(iterations), d_flow_map, dims_out[0], dims_out[1], dims_out[2], steps,
+                           t->stop_rule);
     }, cfg.field[0].layout);
     launch_end(ctx);
   }
