@@ -47,23 +47,6 @@ __device__ __forceinline__ float dec16(uint32_t h, int storage) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h & 0xffffu)));
 }
 
-// One node of a block, decoded to FP32 (x, y, z); the 4th component is
-// padding.
-template <int S>
-__device__ __forceinline__ void load_node(const void* base, uint64_t i, float& x, float& y, float& z) {
-  if constexpr (S == PA_STORAGE_FP32) {
-    const float4 v = static_cast<const float4*>(base)[i];
-    x = v.x; y = v.y; z = v.z;
-  } else {
-    const uint2 u = static_cast<const uint2*>(base)[i];
-    if constexpr (S == PA_STORAGE_BF16) {
-      x = bf16_lo(u.x); y = bf16_hi(u.x); z = bf16_lo(u.y);
-    } else {
-      x = f16_lo(u.x); y = f16_hi(u.x); z = f16_lo(u.y);
-    }
-  }
-}
-
 // Correctly rounded a / h from y = RN(1/h) without the division sequence.
 // q0 = RN(a y) is within 1.5 ulp of a/h; one FMA correction brings it within
 // one ulp, and a second is then RN(a/h) by Markstein's theorem (the residual
@@ -232,70 +215,82 @@ __device__ __forceinline__ void lerp_planar(const FieldDesc& f, const float d[3]
   }
 }
 
-// vector_field::interpolate (vector_field.cpp:22-48): m = floor(q), w =
-// fmod(d, h) / h, lerp (linear_interpolate.hpp:6-10) along z, y, x.  With
-// INTERP == PA_INTERP_CONSISTENT the weight is q - floor(q) (exact in FP32
-// for 0 <= q < 2^23).
-template <int S, int INTERP>
-__device__ __forceinline__ void interpolate(const FieldDesc& f, const float d[3], const float q[3], float v[3]) {
-  // Cell subscripts first so the gathers are in flight while the weights
-  // are computed.
-  int m[3];
+// The loads of one cell's 8 corners as issued (gather_issue), decoded later
+// (gather_corners), so that a caller can place other memory operations
+// between the issue and the first use.  Cell records: the rec_used<S>()
+// uint4 of the record; nodes: the 8 nodes c000 c001 c010 c011 c100 c101
+// c110 c111 (FP32 as uint4, 16-bit in .x .y).
+template <int S>
+struct Gather {
+  uint4 r[8];
+};
+
+template <int S>
+__device__ __forceinline__ void gather_record(const uint4* r, Gather<S>& g) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) m[i] = static_cast<int>(floorf(q[i]));
-  float a0, a1, a2, b0, b1, b2, c0, c1, c2, e0, e1, e2;
-  float g0, g1, g2, h0, h1, h2, k0, k1, k2, l0, l1, l2;
+  for (int i = 0; i < rec_used<S>(); ++i) g.r[i] = gload<uint4>(r, i);
+}
+
+template <int S>
+__device__ __forceinline__ void gather_issue(const FieldDesc& f, const int m[3], Gather<S>& g) {
   if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
-    // One record holds x of c000 c001 c010 c011 c100 c101 c110 c111, then
-    // y, then z.
-    // fewer than 2^31 records (checked when the layout is built)
-    const uint32_t ci = uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
-    if constexpr (storage_of(S) == PA_STORAGE_FP32) {
-      const float4* r = static_cast<const float4*>(f.cells) + uint64_t(ci) * 8;
-      const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
-      a0 = r0.x; b0 = r0.y; c0 = r0.z; e0 = r0.w; g0 = r1.x; h0 = r1.y; k0 = r1.z; l0 = r1.w;
-      a1 = r2.x; b1 = r2.y; c1 = r2.z; e1 = r2.w; g1 = r3.x; h1 = r3.y; k1 = r3.z; l1 = r3.w;
-      a2 = r4.x; b2 = r4.y; c2 = r4.z; e2 = r4.w; g2 = r5.x; h2 = r5.y; k2 = r5.z; l2 = r5.w;
-    } else {
-      constexpr int st = storage_of(S);
-      const uint4* r = static_cast<const uint4*>(f.cells) + uint64_t(ci) * 4;
-      const uint4 r0 = r[0], r1 = r[1], r2 = r[2];
-      a0 = dec16(r0.x, st); b0 = dec16(r0.x >> 16, st); c0 = dec16(r0.y, st); e0 = dec16(r0.y >> 16, st);
-      g0 = dec16(r0.z, st); h0 = dec16(r0.z >> 16, st); k0 = dec16(r0.w, st); l0 = dec16(r0.w >> 16, st);
-      a1 = dec16(r1.x, st); b1 = dec16(r1.x >> 16, st); c1 = dec16(r1.y, st); e1 = dec16(r1.y >> 16, st);
-      g1 = dec16(r1.z, st); h1 = dec16(r1.z >> 16, st); k1 = dec16(r1.w, st); l1 = dec16(r1.w >> 16, st);
-      a2 = dec16(r2.x, st); b2 = dec16(r2.x >> 16, st); c2 = dec16(r2.y, st); e2 = dec16(r2.y >> 16, st);
-      g2 = dec16(r2.z, st); h2 = dec16(r2.z >> 16, st); k2 = dec16(r2.w, st); l2 = dec16(r2.w >> 16, st);
-    }
+    gather_record<S>(cell_record<S>(f, m), g);
   } else {
-    constexpr int SN = storage_of(S);
     // fewer than 2^32 nodes (checked when the field is made)
     const uint32_t s0 = uint32_t(f.stride0), s1 = uint32_t(f.stride1);
     const uint32_t i000 = uint32_t(m[0]) * s0 + uint32_t(m[1]) * s1 + uint32_t(m[2]);
-    const uint32_t i010 = i000 + s1;
-    const uint32_t i100 = i000 + s0;
-    const uint32_t i110 = i100 + s1;
-    load_node<SN>(f.data, i000, a0, a1, a2);
-    load_node<SN>(f.data, i000 + 1, b0, b1, b2);
-    load_node<SN>(f.data, i010, c0, c1, c2);
-    load_node<SN>(f.data, i010 + 1, e0, e1, e2);
-    load_node<SN>(f.data, i100, g0, g1, g2);
-    load_node<SN>(f.data, i100 + 1, h0, h1, h2);
-    load_node<SN>(f.data, i110, k0, k1, k2);
-    load_node<SN>(f.data, i110 + 1, l0, l1, l2);
+    const uint32_t at[8] = {i000, i000 + 1, i000 + s1, i000 + s1 + 1, i000 + s0, i000 + s0 + 1, i000 + s0 + s1,
+                            i000 + s0 + s1 + 1};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if constexpr (storage_of(S) == PA_STORAGE_FP32) {
+        g.r[j] = static_cast<const uint4*>(f.data)[at[j]];
+      } else {
+        const uint2 u = static_cast<const uint2*>(f.data)[at[j]];
+        g.r[j].x = u.x;
+        g.r[j].y = u.y;
+      }
+    }
   }
-  float w[3], u[3];
-  cell_weights<INTERP>(f, d, q, m, w, u);
-  // c00 = lerp(c000, c001, w2), c01 = lerp(c010, c011, w2), ...
-  const float c00x = lerp1(a0, b0, w[2], u[2]), c00y = lerp1(a1, b1, w[2], u[2]), c00z = lerp1(a2, b2, w[2], u[2]);
-  const float c01x = lerp1(c0, e0, w[2], u[2]), c01y = lerp1(c1, e1, w[2], u[2]), c01z = lerp1(c2, e2, w[2], u[2]);
-  const float c10x = lerp1(g0, h0, w[2], u[2]), c10y = lerp1(g1, h1, w[2], u[2]), c10z = lerp1(g2, h2, w[2], u[2]);
-  const float c11x = lerp1(k0, l0, w[2], u[2]), c11y = lerp1(k1, l1, w[2], u[2]), c11z = lerp1(k2, l2, w[2], u[2]);
-  const float c0x = lerp1(c00x, c01x, w[1], u[1]), c0y = lerp1(c00y, c01y, w[1], u[1]), c0z = lerp1(c00z, c01z, w[1], u[1]);
-  const float c1x = lerp1(c10x, c11x, w[1], u[1]), c1y = lerp1(c10y, c11y, w[1], u[1]), c1z = lerp1(c10z, c11z, w[1], u[1]);
-  v[0] = lerp1(c0x, c1x, w[0], u[0]);
-  v[1] = lerp1(c0y, c1y, w[0], u[0]);
-  v[2] = lerp1(c0z, c1z, w[0], u[0]);
+}
+
+// The 24 corner values, component-planar: k[comp * 8 + corner].
+template <int S>
+__device__ __forceinline__ void gather_corners(const Gather<S>& g, float k[24]) {
+  if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
+    decode_record<S>(g.r, k);
+  } else {
+    constexpr int st = storage_of(S);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if constexpr (st == PA_STORAGE_FP32) {
+        k[j] = __uint_as_float(g.r[j].x);
+        k[8 + j] = __uint_as_float(g.r[j].y);
+        k[16 + j] = __uint_as_float(g.r[j].z);
+      } else {
+        k[j] = dec16(g.r[j].x, st);
+        k[8 + j] = dec16(g.r[j].x >> 16, st);
+        k[16 + j] = dec16(g.r[j].y, st);
+      }
+    }
+  }
+}
+
+// vector_field::interpolate (vector_field.cpp:22-48): m = floor(q), w =
+// fmod(d, h) / h, lerp (linear_interpolate.hpp:6-10) along z, y, x.  With
+// INTERP == PA_INTERP_CONSISTENT the weight is q - floor(q) (exact in FP32
+// for 0 <= q < 2^23).  The gathers are issued first so that they are in
+// flight while the weights are computed.
+template <int S, int INTERP>
+__device__ __forceinline__ void interpolate(const FieldDesc& f, const float d[3], const float q[3], float v[3]) {
+  int m[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) m[i] = static_cast<int>(floorf(q[i]));
+  Gather<S> g;
+  gather_issue<S>(f, m, g);
+  float k[24];
+  gather_corners<S>(g, k);
+  lerp_planar<INTERP>(f, d, q, k, v);
 }
 
 // Stop test (particle_tracer.cpp:312): Eigen isZero() with the float
@@ -413,6 +408,32 @@ __device__ __forceinline__ int advance_rt(const TraceConfig& c, const FieldDesc&
   if (!locate(f, last.x, last.y, last.z, d, q)) return 1;
   interpolate<S, INTERP>(f, d, q, v);
   return finish_step<S, MODE, INTERP, STOP_RUNTIME>(c, f, last, v, next, rule);
+}
+
+// Raw buffer stores whose descriptor (out_rsrc) covers byte offsets below
+// kDrop: a lane with nothing to store passes kDrop and the hardware drops
+// the store, so the whole wave issues it without a branch (the same wait
+// counts on every path) and without traffic for the dropped lanes.
+constexpr int kDrop = 0x7FFFFFF0;
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t out_rsrc(void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, kDrop, 0x00020000);
+}
+__device__ __forceinline__ void store16(rsrc_t r, float4 v, int off) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const u4 d = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+  __builtin_amdgcn_raw_buffer_store_b128(d, r, off, 0, 0);
+}
+__device__ __forceinline__ void store4(rsrc_t r, int32_t v, int off) {
+  __builtin_amdgcn_raw_buffer_store_b32(uint32_t(v), r, off, 0, 0);
+}
+
+// A value the whole wave shares, held in scalar registers (a buffer
+// descriptor built from it then needs no per-lane loop).
+__device__ __forceinline__ int64_t uniform(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x)));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x) >> 32));
+  return int64_t((uint64_t(hi) << 32) | lo);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on

@@ -17,6 +17,7 @@
 //             curve (no index i+1) unless i is the last index; other entries
 //             are left as they were, like the colour stage.
 #include "pa_color.hpp"
+#include "pa_device.hpp"
 
 namespace pa_amd {
 namespace {
@@ -50,34 +51,13 @@ __device__ __forceinline__ float4 dpp_f4(float4 v) {
   return make_float4(dpp_f<CTRL>(v.x), dpp_f<CTRL>(v.y), dpp_f<CTRL>(v.z), dpp_f<CTRL>(v.w));
 }
 
-// Output stores are raw buffer stores whose descriptor starts at the group's
-// first output element: a lane with nothing to store gives an offset past
-// the descriptor's range and the hardware drops the store, so every store
-// is issued by the whole wave without branches (exact wait counts) and
-// without traffic of its own.  Offsets are below 2^31 bytes (at most 8
-// seeds of at most 2^24 slots each).
-constexpr int kDrop = 0x7FFFFFF0;
+// Output stores are raw buffer stores (store16 / store4) whose descriptor
+// starts at the group's first output element: a lane with nothing to store
+// gives the offset kDrop and the hardware drops the store, so every store is
+// issued by the whole wave without branches (exact wait counts) and without
+// traffic of its own.  Offsets are below 2^31 bytes (at most 8 seeds of at
+// most 2^24 slots each).
 constexpr int64_t kMaxRegion = int64_t(1) << 24;
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-__device__ __forceinline__ rsrc_t out_rsrc(void* base) {
-  return __builtin_amdgcn_make_buffer_rsrc(base, 0, kDrop, 0x00020000);
-}
-__device__ __forceinline__ void store16(rsrc_t r, float4 v, int off) {
-  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-  const u4 d = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
-  __builtin_amdgcn_raw_buffer_store_b128(d, r, off, 0, 0);
-}
-__device__ __forceinline__ void store4(rsrc_t r, int32_t v, int off) {
-  __builtin_amdgcn_raw_buffer_store_b32(uint32_t(v), r, off, 0, 0);
-}
-
-// A value the whole wave shares, held in scalar registers (the buffer
-// descriptors built from it then need no per-lane loop).
-__device__ __forceinline__ int64_t uniform(int64_t x) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x)));
-  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x) >> 32));
-  return int64_t((uint64_t(hi) << 32) | lo);
-}
 
 // What one lane of a packing wave knows about its seed p (lane 8k + j of the wave
 // holds slot s0 + j of seed k of the group).

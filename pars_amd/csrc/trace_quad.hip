@@ -93,51 +93,40 @@ __device__ __forceinline__ void load_lane(const QEntry (*tab)[4], const QBlock* 
   L.mask = k.mask;
 }
 
-// contains + interpolate of vector_field.cpp:12-48 at the quad's point
-// (lane c: component x); v = component c of the velocity (0 on the w lane).
-// Quad-uniform result; false = outside the block.
-template <int S, int INTERP>
-__device__ __forceinline__ bool quad_eval(const QLane& L, int c, float x, float& v) {
-  const bool wl = c == 3;
-  const float d = x - L.off;
-  const bool fast = quad_all(wl | div_range(d)) & (L.fast != 0);
-  float q;
-  if (__builtin_expect(fast, 1)) q = div_corrected(d, L.h, L.inv);
-  else q = d / L.h;
-  const float s = floorf(q);
-  if (!quad_all(wl | ((s >= 0.0f) & (s <= L.hi)))) return false;
-  const uint32_t m = wl ? 0u : static_cast<uint32_t>(static_cast<int>(s));
-  float k0, k1, k2, k3, k4, k5, k6, k7;  // component c of c000 c001 c010 c011 c100 c101 c110 c111
-  if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
-    const uint32_t ci = quad_sum(m * L.stride);
-    if constexpr (storage_of(S) == PA_STORAGE_FP32) {
-      const uint64_t r = uint64_t(ci) * 8 + 2 * c;
-      const float4 A = gload<float4>(L.data, r), B = gload<float4>(L.data, r + 1);
-      k0 = A.x; k1 = A.y; k2 = A.z; k3 = A.w; k4 = B.x; k5 = B.y; k6 = B.z; k7 = B.w;
-    } else {
-      constexpr int st = storage_of(S);
-      const uint4 U = gload<uint4>(L.data, uint64_t(ci) * 4 + c);
-      k0 = dec16(U.x, st); k1 = dec16(U.x >> 16, st); k2 = dec16(U.y, st); k3 = dec16(U.y >> 16, st);
-      k4 = dec16(U.z, st); k5 = dec16(U.z >> 16, st); k6 = dec16(U.w, st); k7 = dec16(U.w >> 16, st);
-    }
+// Component c's part of cell record ci: FP32 chunks 2c and 2c + 1 (A, B);
+// 16-bit chunk c (A, as bits; B untouched).
+template <int S>
+__device__ __forceinline__ void load_record(const void* data, int c, uint32_t ci, float4& A, float4& B) {
+  if constexpr (storage_of(S) == PA_STORAGE_FP32) {
+    const uint64_t r = uint64_t(ci) * 8 + 2 * c;
+    A = gload<float4>(data, r);
+    B = gload<float4>(data, r + 1);
   } else {
-    const uint32_t n0 = quad_sum(m * L.stride);
-    const uint32_t n2 = n0 + L.st1, n4 = n0 + L.st0, n6 = n4 + L.st1;
-    if constexpr (storage_of(S) == PA_STORAGE_FP32) {
-      const void* D = L.data;
-      k0 = gload<float>(D, uint64_t(n0) * 4 + c); k1 = gload<float>(D, uint64_t(n0 + 1) * 4 + c);
-      k2 = gload<float>(D, uint64_t(n2) * 4 + c); k3 = gload<float>(D, uint64_t(n2 + 1) * 4 + c);
-      k4 = gload<float>(D, uint64_t(n4) * 4 + c); k5 = gload<float>(D, uint64_t(n4 + 1) * 4 + c);
-      k6 = gload<float>(D, uint64_t(n6) * 4 + c); k7 = gload<float>(D, uint64_t(n6 + 1) * 4 + c);
-    } else {
-      constexpr int st = storage_of(S);
-      const void* D = L.data;
-      k0 = dec16(gload<uint16_t>(D, uint64_t(n0) * 4 + c), st); k1 = dec16(gload<uint16_t>(D, uint64_t(n0 + 1) * 4 + c), st);
-      k2 = dec16(gload<uint16_t>(D, uint64_t(n2) * 4 + c), st); k3 = dec16(gload<uint16_t>(D, uint64_t(n2 + 1) * 4 + c), st);
-      k4 = dec16(gload<uint16_t>(D, uint64_t(n4) * 4 + c), st); k5 = dec16(gload<uint16_t>(D, uint64_t(n4 + 1) * 4 + c), st);
-      k6 = dec16(gload<uint16_t>(D, uint64_t(n6) * 4 + c), st); k7 = dec16(gload<uint16_t>(D, uint64_t(n6 + 1) * 4 + c), st);
-    }
+    const uint4 U = gload<uint4>(data, uint64_t(ci) * 4 + c);
+    A = make_float4(__uint_as_float(U.x), __uint_as_float(U.y), __uint_as_float(U.z), __uint_as_float(U.w));
   }
+}
+
+// Component c of the 8 corners c000 c001 c010 c011 c100 c101 c110 c111
+// from the lane's part of a record.
+template <int S>
+__device__ __forceinline__ void decode_quad(const float4& A, const float4& B, float k[8]) {
+  if constexpr (storage_of(S) == PA_STORAGE_FP32) {
+    k[0] = A.x; k[1] = A.y; k[2] = A.z; k[3] = A.w; k[4] = B.x; k[5] = B.y; k[6] = B.z; k[7] = B.w;
+  } else {
+    constexpr int st = storage_of(S);
+    const uint32_t u0 = __float_as_uint(A.x), u1 = __float_as_uint(A.y), u2 = __float_as_uint(A.z),
+                   u3 = __float_as_uint(A.w);
+    k[0] = dec16(u0, st); k[1] = dec16(u0 >> 16, st); k[2] = dec16(u1, st); k[3] = dec16(u1 >> 16, st);
+    k[4] = dec16(u2, st); k[5] = dec16(u2 >> 16, st); k[6] = dec16(u3, st); k[7] = dec16(u3 >> 16, st);
+  }
+}
+
+// Weights (vector_field.cpp:27-31) and the lerp tree (:41-47) over the
+// quad: lane c weighs along its axis; the result is component c of the
+// velocity (0 on the w lane).
+template <int INTERP>
+__device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, float q, float s, const float k[8]) {
   float w;
   if constexpr (INTERP == PA_INTERP_CONSISTENT) {
     w = q - s;
@@ -148,12 +137,68 @@ __device__ __forceinline__ bool quad_eval(const QLane& L, int c, float x, float&
   }
   const float w0 = qmovf<kBcast0>(w), w1 = qmovf<kBcast1>(w), w2 = qmovf<kBcast2>(w);
   const float u0 = 1.0f - w0, u1 = 1.0f - w1, u2 = 1.0f - w2;
-  const float c00 = lerp1(k0, k1, w2, u2), c01 = lerp1(k2, k3, w2, u2);
-  const float c10 = lerp1(k4, k5, w2, u2), c11 = lerp1(k6, k7, w2, u2);
+  const float c00 = lerp1(k[0], k[1], w2, u2), c01 = lerp1(k[2], k[3], w2, u2);
+  const float c10 = lerp1(k[4], k[5], w2, u2), c11 = lerp1(k[6], k[7], w2, u2);
   const float c0 = lerp1(c00, c01, w1, u1), c1 = lerp1(c10, c11, w1, u1);
-  v = wl ? 0.0f : lerp1(c0, c1, w0, u0);
+  return wl ? 0.0f : lerp1(c0, c1, w0, u0);
+}
+
+// The quotient of vector_field::contains (vector_field.cpp:12-21) along
+// the lane's axis: d = x - offset, q = d / h, s = floor(q); quad-uniform
+// result, false = outside the block.
+__device__ __forceinline__ bool quad_locate(const QLane& L, bool wl, float x, float& d, float& q, float& s) {
+  d = x - L.off;
+  const bool fast = quad_all(wl | div_range(d)) & (L.fast != 0);
+  if (__builtin_expect(fast, 1)) q = div_corrected(d, L.h, L.inv);
+  else q = d / L.h;
+  s = floorf(q);
+  return quad_all(wl | ((s >= 0.0f) & (s <= L.hi)));
+}
+
+// contains + interpolate of vector_field.cpp:12-48 at the quad's point
+// (lane c: component x); v = component c of the velocity (0 on the w lane).
+// Quad-uniform result; false = outside the block.
+template <int S, int INTERP>
+__device__ __forceinline__ bool quad_eval(const QLane& L, int c, float x, float& v) {
+  const bool wl = c == 3;
+  float d, q, s;
+  if (!quad_locate(L, wl, x, d, q, s)) return false;
+  const uint32_t m = wl ? 0u : static_cast<uint32_t>(static_cast<int>(s));
+  float k[8];  // component c of c000 c001 c010 c011 c100 c101 c110 c111
+  if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
+    float4 A, B;
+    load_record<S>(L.data, c, quad_sum(m * L.stride), A, B);
+    decode_quad<S>(A, B, k);
+  } else {
+    const uint32_t n0 = quad_sum(m * L.stride);
+    const uint32_t n2 = n0 + L.st1, n4 = n0 + L.st0, n6 = n4 + L.st1;
+    if constexpr (storage_of(S) == PA_STORAGE_FP32) {
+      const void* D = L.data;
+      k[0] = gload<float>(D, uint64_t(n0) * 4 + c); k[1] = gload<float>(D, uint64_t(n0 + 1) * 4 + c);
+      k[2] = gload<float>(D, uint64_t(n2) * 4 + c); k[3] = gload<float>(D, uint64_t(n2 + 1) * 4 + c);
+      k[4] = gload<float>(D, uint64_t(n4) * 4 + c); k[5] = gload<float>(D, uint64_t(n4 + 1) * 4 + c);
+      k[6] = gload<float>(D, uint64_t(n6) * 4 + c); k[7] = gload<float>(D, uint64_t(n6 + 1) * 4 + c);
+    } else {
+      constexpr int st = storage_of(S);
+      const void* D = L.data;
+      k[0] = dec16(gload<uint16_t>(D, uint64_t(n0) * 4 + c), st); k[1] = dec16(gload<uint16_t>(D, uint64_t(n0 + 1) * 4 + c), st);
+      k[2] = dec16(gload<uint16_t>(D, uint64_t(n2) * 4 + c), st); k[3] = dec16(gload<uint16_t>(D, uint64_t(n2 + 1) * 4 + c), st);
+      k[4] = dec16(gload<uint16_t>(D, uint64_t(n4) * 4 + c), st); k[5] = dec16(gload<uint16_t>(D, uint64_t(n4 + 1) * 4 + c), st);
+      k[6] = dec16(gload<uint16_t>(D, uint64_t(n6) * 4 + c), st); k[7] = dec16(gload<uint16_t>(D, uint64_t(n6 + 1) * 4 + c), st);
+    }
+  }
+  v = quad_lerp<INTERP>(L, wl, d, q, s, k);
   return true;
 }
+
+// A quad's record for this iteration, gathered before the quad's state
+// machine runs (cell records): in = the quad steps and its point is inside
+// the block; d, q, s as quad_locate; A, B as load_record.
+struct QGather {
+  bool in;
+  float d, q, s;
+  float4 A, B;
+};
 
 // Stop test of the lane engine (stops<>), quad-uniform.
 __device__ __forceinline__ bool quad_stops(float v, int c, int rule, float eps2) {
@@ -169,12 +214,11 @@ __device__ __forceinline__ float frozen_rk4(const TraceConfig& k, float x, float
   return (((x + k.a0 * v) + k.a1 * v) + k.a2 * v) + k.a3 * v;
 }
 
-// One step of advance<> (0 stepped, 1 left the block, 2 stop rule).
+// The stop test and the step of advance<> once v = field(x) is known (0
+// stepped, 2 stop rule).
 template <int S, int MODE, int INTERP>
-__device__ __forceinline__ int quad_advance(const TraceConfig& k, const QLane& L, int c, float x, float& next,
-                                            int rule) {
-  float v;
-  if (!quad_eval<S, INTERP>(L, c, x, v)) return 1;
+__device__ __forceinline__ int quad_finish(const TraceConfig& k, const QLane& L, int c, float x, float v, float& next,
+                                           int rule) {
   if (quad_stops(v, c, rule, k.stop_eps2)) return 2;
   if constexpr (MODE == STEP_EULER) {
     next = x + k.dt * v;
@@ -188,6 +232,23 @@ __device__ __forceinline__ int quad_advance(const TraceConfig& k, const QLane& L
     next = frozen_rk4(k, x, v);
   }
   return 0;
+}
+
+// One step of advance<> (0 stepped, 1 left the block, 2 stop rule); with
+// G the record was gathered before (QGather).
+template <int S, int MODE, int INTERP>
+__device__ __forceinline__ int quad_advance(const TraceConfig& k, const QLane& L, int c, float x, float& next,
+                                            int rule, const QGather* G = nullptr) {
+  float v;
+  if (G) {
+    if (!G->in) return 1;
+    float kc[8];
+    decode_quad<S>(G->A, G->B, kc);
+    v = quad_lerp<INTERP>(L, c == 3, G->d, G->q, G->s, kc);
+  } else if (!quad_eval<S, INTERP>(L, c, x, v)) {
+    return 1;
+  }
+  return quad_finish<S, MODE, INTERP>(k, L, c, x, v, next, rule);
 }
 
 // face_rule of the lane engine over the quad: first of x<min, x>max, y<min,
@@ -269,11 +330,12 @@ __device__ __forceinline__ void write_continuation(pa_particle* dst, const float
 
 // One iteration of an active quad (Q_START or Q_STEP): produces exactly one
 // slot, returned as this lane's component; the slot index is st.slot - 1
-// afterwards.  Segment logic of the lane engine's trace_union.
+// afterwards.  Segment logic of the lane engine's trace_union.  G: the
+// record of this iteration, gathered before (see quad_advance).
 template <int S, int MODE, int INTERP>
 __device__ __forceinline__ float quad_iterate(const TraceConfig& cfg, const QEntry (*tab)[4], const QBlock* blk,
                                               QState& st, QLane& L, float& last, int c, int rule, int64_t region,
-                                              pa_particle* cont, unsigned* n_cont) {
+                                              pa_particle* cont, unsigned* n_cont, const QGather* G = nullptr) {
   const float term = c == 3 ? 0.0f : -2.0f;  // termination vertex (-2, -2, -2, 0)
   float val = term;
   ++st.slot;
@@ -289,7 +351,7 @@ __device__ __forceinline__ float quad_iterate(const TraceConfig& cfg, const QEnt
     st.state = Q_DONE;
   } else {
     float next;
-    const int a = quad_advance<S, MODE, INTERP>(cfg, L, c, last, next, rule);
+    const int a = quad_advance<S, MODE, INTERP>(cfg, L, c, last, next, rule, G);
     if (a == 0) {
       val = next;
       last = next;
@@ -426,6 +488,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
 // lane) and finish there.  Both modes run the same per-iteration segment
 // machine, so the slots are those of the lane engine, bit for bit.
 constexpr int kQuadSwitch = 16;
+constexpr int64_t kHybridMaxRegion = int64_t(1) << 21;  // the lane mode's store offsets stay below kDrop
 
 template <int S, int MODE, int INTERP>
 __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
@@ -477,10 +540,62 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   }
 
   // ---- lane mode
+  // The vertex of an iteration is stored in the next one, right after that
+  // iteration's gather is issued: memory operations retire in order, so the
+  // gather's wait then leaves the store in flight instead of waiting for it.
+  // The store is issued by the whole wave (store16 with kDrop for lanes with
+  // nothing pending), so the wait counts are the same on every path.  All
+  // stores of a wave fall in its 8 seed groups: byte offsets below 1024 *
+  // region (region <= 2^21, checked at launch).
+  const int64_t wave0 = logical_block(cfg.xcd_remap) * kQuadBlock + (threadIdx.x & ~63);
+  const int64_t vbase = group8_slot(wave0, 0, region);
+  const rsrc_t vrs = out_rsrc(verts + uniform(vbase));
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // the set-up loads, once (see the hand-off below)
+  int pend = kDrop;
+  float4 pval = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   uint64_t live;
   for (;;) {
     live = __ballot(st.state != Q_IDLE);
     if (__popcll(live) <= kQuadSwitch) break;
+    // advance_rt of a stepping lane, split around the store: a = 0 stepped
+    // (next valid), 1 left the block, 2 stop rule.  Every lane issues the
+    // gather, so its registers need no merge of paths (whose copies would
+    // wait for it).  A lane that does not step gathers a valid stand-in: with
+    // cell records the record of the wave's first stepping lane (no extra
+    // lines), with nodes the cell of its own position clamped into the block.
+    float d[3], q[3];
+    const bool inside = (int(locate(f, last.x, last.y, last.z, d, q)) &
+                         int(st.state == Q_STEP && st.it != st.R - 1)) != 0;
+    int m[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)  // floor(q) for a lane inside the block
+      m[i] = static_cast<int>(fminf(fmaxf(floorf(q[i]), 0.0f), static_cast<float>(f.dims[i] - 2)));
+    Gather<S> g;
+    if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
+      uint64_t r = reinterpret_cast<uint64_t>(cell_record<S>(f, m));
+      const uint64_t W = __ballot(inside);
+      if (W != 0 && !inside) {
+        const int first = __builtin_ctzll(W);
+        // readlane returns int: both halves go through uint32_t so that the
+        // low half is not sign-extended into the high one
+        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r >> 32), first));
+        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(uint32_t(r)), first));
+        r = uint64_t(hi) << 32 | lo;
+      }
+      gather_record<S>(reinterpret_cast<const uint4*>(r), g);
+    } else {
+      gather_issue<S>(f, m, g);
+    }
+    store16(vrs, pval, pend);
+    pend = kDrop;
+    int a = 1;
+    float4 next;
+    if (inside) {
+      float k[24], v[3];
+      gather_corners<S>(g, k);
+      lerp_planar<INTERP>(f, d, q, k, v);
+      a = finish_step<S, MODE, INTERP, STOP_RUNTIME>(cfg, f, last, v, next, rule);
+    }
     if (st.state != Q_IDLE) {
       float4 val = termination_vertex();
       ++st.slot;
@@ -495,8 +610,6 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       } else if (st.it == st.R - 1) {
         st.state = Q_DONE;
       } else {
-        float4 next;
-        const int a = advance_rt<S, MODE, INTERP>(cfg, f, last, next, rule);
         if (a == 0) {
           val = next;
           last = next;
@@ -522,13 +635,18 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
                 st.b = nb;
                 st.R = Rn;
                 f = cfg.field[st.b + 1];
+                // wait for the descriptor here (a rare path): otherwise its
+                // registers count as pending at the loop head of every
+                // iteration, whose wait would then take the vertex store too
+                __builtin_amdgcn_s_waitcnt(0x0F70);
                 st.state = Q_START;
               }
             }
           }
         }
       }
-      verts[group8_slot(st.pid, st.slot - 1, region)] = val;
+      pend = static_cast<int>((group8_slot(st.pid, st.slot - 1, region) - vbase) * 16);
+      pval = val;
       const bool isT = is_termination_vertex(val);
       if (st.slot > 1 && !st.prevT && !isT) ++st.pairs;
       st.prevT = isT;
@@ -542,6 +660,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       }
     }
   }
+
+  store16(vrs, pval, pend);
 
   // ---- quad mode: live particle k of the wave goes to quad k
   const int nlive = __popcll(live);
@@ -568,6 +688,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     QState qs;
     float ql = 0.0f;
     QLane L;
+    load_lane(tab, blk, -1, c, L);  // every quad gathers (below): a valid block for the idle ones
     qs.state = Q_IDLE;
     if (q < nlive) {
       const float4 xl = x_last[w][q];
@@ -585,12 +706,31 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       qs.prevT = xi[9];
       load_lane(tab, blk, qs.b, c, L);
     }
-    float* vf = reinterpret_cast<float*>(verts);
+    // As in the lane mode, a quad's vertex component is stored in the next
+    // iteration, right after that iteration's gather; with cell records the
+    // gather runs before the state machine, every quad issuing it (a quad
+    // that does not step gathers its own cell clamped into the block).
+    constexpr bool kCells = layout_of(S) == PA_LAYOUT_CELLS;
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // the lane mode's memory operations, once
+    int qpend = kDrop;
+    float qval = 0.0f;
     for (;;) {
       if (__ballot(qs.state != Q_IDLE) == 0) break;
+      QGather G;
+      if constexpr (kCells) {
+        const bool wl = c == 3;
+        G.in = quad_locate(L, wl, ql, G.d, G.q, G.s) & (qs.state == Q_STEP) & (qs.it != qs.R - 1);
+        const float sc = fminf(fmaxf(G.s, 0.0f), L.hi);  // G.s for a quad inside the block
+        const uint32_t ci = quad_sum(wl ? 0u : static_cast<uint32_t>(static_cast<int>(sc)) * L.stride);
+        load_record<S>(L.data, c, ci, G.A, G.B);
+      }
+      store4(vrs, __float_as_int(qval), qpend);
+      qpend = kDrop;
       if (qs.state != Q_IDLE) {
-        const float val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont);
-        vf[group8_slot(qs.pid, qs.slot - 1, region) * 4 + c] = val;
+        const float val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont,
+                                                        kCells ? &G : nullptr);
+        qpend = static_cast<int>((group8_slot(qs.pid, qs.slot - 1, region) - vbase) * 16 + 4 * c);
+        qval = val;
         const bool isT = quad_all(val == (c == 3 ? 0.0f : -2.0f));
         if (qs.slot > 1 && !qs.prevT && !isT) ++qs.pairs;
         qs.prevT = isT;
@@ -606,6 +746,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         }
       }
     }
+    store4(vrs, __float_as_int(qval), qpend);
   }
   wave_steps = wave_sum_u64(wave_steps);
   if (lane == 0 && wave_steps) atomicAdd(steps_out, wave_steps);
@@ -616,6 +757,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
 pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
                               unsigned long long* steps, unsigned* ncont, int32_t* pairs) {
   if (l.n > INT32_MAX) return fail(ctx, PA_E_INVALID, "hybrid engine: at most 2^31-1 seeds per call");
+  if (l.region > kHybridMaxRegion) return fail(ctx, PA_E_INVALID, "hybrid engine: region_slots at most 2^21");
   const int64_t blocks = (l.n + kQuadBlock - 1) / kQuadBlock;
   dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
     hipLaunchKernelGGL((k_hybrid_union<decltype(S)::value, decltype(M)::value, decltype(I)::value>),

@@ -324,6 +324,38 @@ def test_union_round_capacity(ctx, orc):
         assert verts.download(np.float32, 4 * nv).tobytes() == sentinel.tobytes()
 
 
+def test_hybrid_region_limit(ctx, orc):
+    """The hybrid engine takes region_slots up to 2^21: one more fails with
+    PA_E_INVALID before anything is written; at the limit the slots equal
+    those of a small region."""
+    d, o, _ = single_block(ctx, orc, 24, layout=pa.LAYOUT_CELLS)
+    t = pa.particle_tracer(ctx)
+    t.set_local_vector_field(d)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.02)
+    t.set_engine(pa.ENGINE_HYBRID)
+    seeds = orc.seed_generate((0, 0, 0), o.size, [f32(12) * o.spacing[0]] * 3, 30, 0)
+    n = len(seeds)
+    assert 0 < n <= 8
+    parts = pa.Particles.from_host(ctx, seeds)
+    limit = 1 << 21
+    small = t.trace_union(parts, 32, order="group8")
+    want = small["vertices"].download(np.float32, 8 * 32 * 4).reshape(32, 8, 4)
+    counts = small["counts"].download(np.int32, n)
+    big = t.union_buffers(n, limit)
+    sentinel = np.full(8 * 64 * 4, 5.5, np.float32)
+    big["vertices"].upload(sentinel)
+    with pytest.raises(pa.PaError) as e:
+        t.trace_union(parts, limit + 1, buffers=big, order="group8")
+    assert e.value.status == pa.E_INVALID
+    assert big["vertices"].download(np.float32, sentinel.size).tobytes() == sentinel.tobytes()
+    r = t.trace_union(parts, limit, buffers=big, order="group8")
+    assert np.array_equal(r["counts"].download(np.int32, n), counts)
+    got = r["vertices"].download(np.float32, 8 * 32 * 4).reshape(32, 8, 4)
+    for p in range(n):
+        assert got[:counts[p], p].tobytes() == want[:counts[p], p].tobytes()
+
+
 def test_prune_index_color_parity(ctx, orc):
     d, o, _ = single_block(ctx, orc, 48)
     h = o.spacing[0]
