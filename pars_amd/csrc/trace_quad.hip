@@ -540,13 +540,16 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   }
 
   // ---- lane mode
-  // The vertex of an iteration is stored in the next one, right after that
-  // iteration's gather is issued: memory operations retire in order, so the
-  // gather's wait then leaves the store in flight instead of waiting for it.
-  // The store is issued by the whole wave (store16 with kDrop for lanes with
-  // nothing pending), so the wait counts are the same on every path.  All
-  // stores of a wave fall in its 8 seed groups: byte offsets below 1024 *
-  // region (region <= 2^21, checked at launch).
+  // Cell records: the vertex of an iteration is stored in the next one,
+  // right after that iteration's gather is issued: memory operations retire
+  // in order, so the gather's wait then leaves the store in flight instead of
+  // waiting for it.  The store is issued by the whole wave (store16 with
+  // kDrop for lanes with nothing pending), so the wait counts are the same on
+  // every path.  All stores of a wave fall in its 8 seed groups: byte offsets
+  // below 1024 * region (region <= 2^21, checked at launch).  Nodes (8
+  // scattered loads per gather) keep the gather inside the step and store at
+  // once.
+  constexpr bool kCells = layout_of(S) == PA_LAYOUT_CELLS;
   const int64_t wave0 = logical_block(cfg.xcd_remap) * kQuadBlock + (threadIdx.x & ~63);
   const int64_t vbase = group8_slot(wave0, 0, region);
   const rsrc_t vrs = out_rsrc(verts + uniform(vbase));
@@ -557,21 +560,21 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   for (;;) {
     live = __ballot(st.state != Q_IDLE);
     if (__popcll(live) <= kQuadSwitch) break;
-    // advance_rt of a stepping lane, split around the store: a = 0 stepped
-    // (next valid), 1 left the block, 2 stop rule.  Every lane issues the
-    // gather, so its registers need no merge of paths (whose copies would
-    // wait for it).  A lane that does not step gathers a valid stand-in: with
-    // cell records the record of the wave's first stepping lane (no extra
-    // lines), with nodes the cell of its own position clamped into the block.
-    float d[3], q[3];
-    const bool inside = (int(locate(f, last.x, last.y, last.z, d, q)) &
-                         int(st.state == Q_STEP && st.it != st.R - 1)) != 0;
-    int m[3];
+    // Cell records: advance_rt of a stepping lane, split around the store
+    // (a = 0 stepped, next valid; 1 left the block; 2 stop rule).  Every lane
+    // issues the gather, so its registers need no merge of paths (whose
+    // copies would wait for it); a lane that does not step gathers the
+    // record of the wave's first stepping lane (no extra lines).
+    int a = 1;
+    float4 next;
+    if constexpr (kCells) {
+      float d[3], q[3];
+      const bool inside = (int(locate(f, last.x, last.y, last.z, d, q)) &
+                           int(st.state == Q_STEP && st.it != st.R - 1)) != 0;
+      int m[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)  // floor(q) for a lane inside the block
-      m[i] = static_cast<int>(fminf(fmaxf(floorf(q[i]), 0.0f), static_cast<float>(f.dims[i] - 2)));
-    Gather<S> g;
-    if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
+      for (int i = 0; i < 3; ++i)  // floor(q) for a lane inside the block
+        m[i] = static_cast<int>(fminf(fmaxf(floorf(q[i]), 0.0f), static_cast<float>(f.dims[i] - 2)));
       uint64_t r = reinterpret_cast<uint64_t>(cell_record<S>(f, m));
       const uint64_t W = __ballot(inside);
       if (W != 0 && !inside) {
@@ -582,19 +585,16 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(uint32_t(r)), first));
         r = uint64_t(hi) << 32 | lo;
       }
+      Gather<S> g;
       gather_record<S>(reinterpret_cast<const uint4*>(r), g);
-    } else {
-      gather_issue<S>(f, m, g);
-    }
-    store16(vrs, pval, pend);
-    pend = kDrop;
-    int a = 1;
-    float4 next;
-    if (inside) {
-      float k[24], v[3];
-      gather_corners<S>(g, k);
-      lerp_planar<INTERP>(f, d, q, k, v);
-      a = finish_step<S, MODE, INTERP, STOP_RUNTIME>(cfg, f, last, v, next, rule);
+      store16(vrs, pval, pend);
+      pend = kDrop;
+      if (inside) {
+        float k[24], v[3];
+        gather_corners<S>(g, k);
+        lerp_planar<INTERP>(f, d, q, k, v);
+        a = finish_step<S, MODE, INTERP, STOP_RUNTIME>(cfg, f, last, v, next, rule);
+      }
     }
     if (st.state != Q_IDLE) {
       float4 val = termination_vertex();
@@ -610,6 +610,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       } else if (st.it == st.R - 1) {
         st.state = Q_DONE;
       } else {
+        if constexpr (!kCells) a = advance_rt<S, MODE, INTERP>(cfg, f, last, next, rule);
         if (a == 0) {
           val = next;
           last = next;
@@ -645,8 +646,12 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
           }
         }
       }
-      pend = static_cast<int>((group8_slot(st.pid, st.slot - 1, region) - vbase) * 16);
-      pval = val;
+      if constexpr (kCells) {
+        pend = static_cast<int>((group8_slot(st.pid, st.slot - 1, region) - vbase) * 16);
+        pval = val;
+      } else {
+        verts[group8_slot(st.pid, st.slot - 1, region)] = val;
+      }
       const bool isT = is_termination_vertex(val);
       if (st.slot > 1 && !st.prevT && !isT) ++st.pairs;
       st.prevT = isT;
@@ -661,7 +666,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     }
   }
 
-  store16(vrs, pval, pend);
+  if constexpr (kCells) store16(vrs, pval, pend);
 
   // ---- quad mode: live particle k of the wave goes to quad k
   const int nlive = __popcll(live);
@@ -710,7 +715,6 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     // iteration, right after that iteration's gather; with cell records the
     // gather runs before the state machine, every quad issuing it (a quad
     // that does not step gathers its own cell clamped into the block).
-    constexpr bool kCells = layout_of(S) == PA_LAYOUT_CELLS;
     __builtin_amdgcn_s_waitcnt(0x0F70);  // the lane mode's memory operations, once
     int qpend = kDrop;
     float qval = 0.0f;
