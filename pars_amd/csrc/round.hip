@@ -343,6 +343,10 @@ __global__ __launch_bounds__(kSplitThreads) void k_group8_split(const float4* __
   }
   __syncthreads();
   if (w == 0) {
+    // The packing wave's progress bounds the kernel (memory latency per
+    // row); the colour waves only need to keep their rings from filling.
+    // Issue priority to the packer: 0.89 -> 0.83 ms on the share round.
+    __builtin_amdgcn_s_setprio(3);
     const int64_t g = int64_t(blockIdx.x);
     if (g * 8 < n) {
       RingSink sink{rings, 0u, 0u, flags, true};
