@@ -322,7 +322,7 @@ typedef enum pa_slot_order {
   /* 8-seed groups, step-major inside a group: slot s of seed p is vec4
    * ((p / 8) * region_slots + s) * 8 + p % 8 (allocate ceil(n / 8) * 8 *
    * region_slots vec4); written by PA_ENGINE_HYBRID only, with
-   * region_slots at most 2^21 (PA_E_INVALID above). */
+   * region_slots below 2^21 (PA_E_INVALID otherwise). */
   PA_SLOTS_GROUP8 = 3
 } pa_slot_order;
 pa_status   pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
@@ -343,7 +343,7 @@ pa_status   pa_gpu_tracer_trace_union_layout(pa_gpu_tracer* tracer, const pa_par
  * are checked on the device: a field index outside -1..5 (PA_E_INVALID), a
  * block that is not loaded (PA_E_STATE), remaining_iterations above
  * region_slots (PA_E_CAPACITY) or negative (PA_E_INVALID) fail the call;
- * region_slots is at most 2^21.  The call reads back once, at its end. */
+ * region_slots is below 2^21.  The call reads back once, at its end. */
 pa_status   pa_gpu_tracer_trace_union_round(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
                                             size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
                                             pa_particle* d_continuation, size_t* n_continuation,

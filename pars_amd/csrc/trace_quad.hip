@@ -132,7 +132,8 @@ __device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, flo
     w = q - s;
   } else {
     const float r = exact_fmod(d, L.h, q);
-    if (__builtin_expect(quad_all(wl | div_range(r)) & (L.fast != 0), 1)) w = div_corrected(r, L.h, L.inv);
+    // per lane: both branches give the correctly rounded quotient
+    if (__builtin_expect((int(wl | div_range(r)) & int(L.fast != 0)) != 0, 1)) w = div_corrected(r, L.h, L.inv);
     else w = r / L.h;
   }
   const float w0 = qmovf<kBcast0>(w), w1 = qmovf<kBcast1>(w), w2 = qmovf<kBcast2>(w);
@@ -148,7 +149,8 @@ __device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, flo
 // result, false = outside the block.
 __device__ __forceinline__ bool quad_locate(const QLane& L, bool wl, float x, float& d, float& q, float& s) {
   d = x - L.off;
-  const bool fast = quad_all(wl | div_range(d)) & (L.fast != 0);
+  // per lane: both branches give the correctly rounded quotient
+  const bool fast = (int(wl | div_range(d)) & int(L.fast != 0)) != 0;
   if (__builtin_expect(fast, 1)) q = div_corrected(d, L.h, L.inv);
   else q = d / L.h;
   s = floorf(q);
@@ -488,7 +490,10 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
 // lane) and finish there.  Both modes run the same per-iteration segment
 // machine, so the slots are those of the lane engine, bit for bit.
 constexpr int kQuadSwitch = 16;
-constexpr int64_t kHybridMaxRegion = int64_t(1) << 21;  // the lane mode's store offsets stay below kDrop
+// A wave's vertex stores fall in its 8 seed groups: byte offsets up to
+// 1024 * region - 4 (the w component of seed 63's last slot), which with
+// region < 2^21 stay below kDrop = 2^31 - 16.
+constexpr int64_t kHybridMaxRegion = (int64_t(1) << 21) - 1;
 
 template <int S, int MODE, int INTERP>
 __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
@@ -545,8 +550,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   // in order, so the gather's wait then leaves the store in flight instead of
   // waiting for it.  The store is issued by the whole wave (store16 with
   // kDrop for lanes with nothing pending), so the wait counts are the same on
-  // every path.  All stores of a wave fall in its 8 seed groups: byte offsets
-  // below 1024 * region (region <= 2^21, checked at launch).  Nodes (8
+  // every path; its offsets stay below kDrop (kHybridMaxRegion).  Nodes (8
   // scattered loads per gather) keep the gather inside the step and store at
   // once.
   constexpr bool kCells = layout_of(S) == PA_LAYOUT_CELLS;
@@ -556,6 +560,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   __builtin_amdgcn_s_waitcnt(0x0F70);  // the set-up loads, once (see the hand-off below)
   int pend = kDrop;
   float4 pval = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  // byte offset of the lane's slot 0 (slot s is 128 B further per slot)
+  const int lbase = static_cast<int>((group8_slot(st.pid, 0, region) - vbase) * 16);
   uint64_t live;
   for (;;) {
     live = __ballot(st.state != Q_IDLE);
@@ -647,7 +653,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         }
       }
       if constexpr (kCells) {
-        pend = static_cast<int>((group8_slot(st.pid, st.slot - 1, region) - vbase) * 16);
+        pend = lbase + (st.slot - 1) * 128;
         pval = val;
       } else {
         verts[group8_slot(st.pid, st.slot - 1, region)] = val;
@@ -695,6 +701,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     QLane L;
     load_lane(tab, blk, -1, c, L);  // every quad gathers (below): a valid block for the idle ones
     qs.state = Q_IDLE;
+    qs.pid = wave0;
     if (q < nlive) {
       const float4 xl = x_last[w][q];
       ql = c == 0 ? xl.x : c == 1 ? xl.y : c == 2 ? xl.z : xl.w;
@@ -718,6 +725,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     __builtin_amdgcn_s_waitcnt(0x0F70);  // the lane mode's memory operations, once
     int qpend = kDrop;
     float qval = 0.0f;
+    // byte offset of component c of the quad's slot 0 (128 B per slot)
+    const int qbase = static_cast<int>((group8_slot(qs.pid, 0, region) - vbase) * 16 + 4 * c);
     for (;;) {
       if (__ballot(qs.state != Q_IDLE) == 0) break;
       QGather G;
@@ -733,7 +742,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       if (qs.state != Q_IDLE) {
         const float val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont,
                                                         kCells ? &G : nullptr);
-        qpend = static_cast<int>((group8_slot(qs.pid, qs.slot - 1, region) - vbase) * 16 + 4 * c);
+        qpend = qbase + (qs.slot - 1) * 128;
         qval = val;
         const bool isT = quad_all(val == (c == 3 ? 0.0f : -2.0f));
         if (qs.slot > 1 && !qs.prevT && !isT) ++qs.pairs;
@@ -761,7 +770,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
 pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
                               unsigned long long* steps, unsigned* ncont, int32_t* pairs) {
   if (l.n > INT32_MAX) return fail(ctx, PA_E_INVALID, "hybrid engine: at most 2^31-1 seeds per call");
-  if (l.region > kHybridMaxRegion) return fail(ctx, PA_E_INVALID, "hybrid engine: region_slots at most 2^21");
+  if (l.region > kHybridMaxRegion) return fail(ctx, PA_E_INVALID, "hybrid engine: region_slots below 2^21");
   const int64_t blocks = (l.n + kQuadBlock - 1) / kQuadBlock;
   dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
     hipLaunchKernelGGL((k_hybrid_union<decltype(S)::value, decltype(M)::value, decltype(I)::value>),

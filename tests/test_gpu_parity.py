@@ -325,8 +325,8 @@ def test_union_round_capacity(ctx, orc):
 
 
 def test_hybrid_region_limit(ctx, orc):
-    """The hybrid engine takes region_slots up to 2^21: one more fails with
-    PA_E_INVALID before anything is written; at the limit the slots equal
+    """The hybrid engine takes region_slots below 2^21: 2^21 fails with
+    PA_E_INVALID before anything is written; at 2^21 - 1 the slots equal
     those of a small region."""
     d, o, _ = single_block(ctx, orc, 24, layout=pa.LAYOUT_CELLS)
     t = pa.particle_tracer(ctx)
@@ -338,7 +338,7 @@ def test_hybrid_region_limit(ctx, orc):
     n = len(seeds)
     assert 0 < n <= 8
     parts = pa.Particles.from_host(ctx, seeds)
-    limit = 1 << 21
+    limit = (1 << 21) - 1
     small = t.trace_union(parts, 32, order="group8")
     want = small["vertices"].download(np.float32, 8 * 32 * 4).reshape(32, 8, 4)
     counts = small["counts"].download(np.int32, n)
