@@ -480,6 +480,62 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
   if (lane == 0 && wave_steps) atomicAdd(steps_out, wave_steps);
 }
 
+// FP32 cell records, lane mode.  The six chunks a lane loads are three
+// from base a and three from base b.  Alone, a lane loads its own record (a
+// = record, b = record + 48 B).  While a wave is full (pair), the two lanes
+// of a pair (l, l ^ 1) share their loads: the even lane loads chunks 0-2 of
+// both records, the odd lane chunks 3-5, so each load instruction touches
+// one record line per pair instead of one per lane: the L1 looks up half as
+// many tags, which is what limits a full wave's gathers.  A DPP swap then
+// hands each lane the half of its own record that its partner loaded.  Both
+// forms use the same six loads, so their registers need no merge of paths.
+struct PairGather {
+  uint4 A[3], B[3];
+};
+
+__device__ __forceinline__ void pair_issue(uint64_t r, bool pair, bool even, PairGather& P) {
+  const uint32_t plo = static_cast<uint32_t>(qmov<kSwap1>(static_cast<int>(uint32_t(r))));
+  const uint32_t phi = static_cast<uint32_t>(qmov<kSwap1>(static_cast<int>(r >> 32)));
+  const uint64_t p = uint64_t(phi) << 32 | plo;
+  const uint64_t a = pair && !even ? p + 48 : r;
+  const uint64_t b = !pair ? r + 48 : even ? p : r + 48;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    P.A[k] = gload<uint4>(reinterpret_cast<const uint4*>(a), k);
+    P.B[k] = gload<uint4>(reinterpret_cast<const uint4*>(b), k);
+  }
+}
+
+__device__ __forceinline__ uint32_t swap_pair(uint32_t v) {
+  return static_cast<uint32_t>(qmov<kSwap1>(static_cast<int>(v)));
+}
+
+__device__ __forceinline__ uint4 sel4(bool c, uint4 x, uint4 y) {  // value select (no lvalue select)
+  return make_uint4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
+}
+
+// The lane's own record (chunks 0-5).
+template <int S>
+__device__ __forceinline__ void pair_finish(const PairGather& P, bool pair, bool even, Gather<S>& g) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    g.r[k] = P.A[k];
+    g.r[3 + k] = P.B[k];
+  }
+  if (pair) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint4 give = sel4(even, P.B[k], P.A[k]);  // what the partner needs
+      const uint4 got = make_uint4(swap_pair(give.x), swap_pair(give.y), swap_pair(give.z), swap_pair(give.w));
+      g.r[k] = sel4(even, P.A[k], got);
+      g.r[3 + k] = sel4(even, got, P.B[k]);
+    }
+  }
+}
+
+// Pair gathers while more than this many lanes of the wave are live.
+constexpr int kPairLive = 32;
+
 // Hybrid union trace into 8-seed grouped slots (PA_SLOTS_GROUP8: slot s of
 // seed p = vec4 at ((p / 8) * region + s) * 8 + p % 8; a wave's stores of one
 // slot fill whole 128-B lines, and a group's slots are contiguous).  Every lane starts with one seed
@@ -592,9 +648,16 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         r = uint64_t(hi) << 32 | lo;
       }
       Gather<S> g;
-      gather_record<S>(reinterpret_cast<const uint4*>(r), g);
+      [[maybe_unused]] PairGather P;
+      const bool pair = __popcll(live) > kPairLive;  // wave-uniform
+      const bool even = (lane & 1) == 0;
+      if constexpr (storage_of(S) == PA_STORAGE_FP32) pair_issue(r, pair, even, P);
+      else gather_record<S>(reinterpret_cast<const uint4*>(r), g);
       store16(vrs, pval, pend);
       pend = kDrop;
+      // keep the exchange (which waits for the loads) after the store
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (storage_of(S) == PA_STORAGE_FP32) pair_finish<S>(P, pair, even, g);
       if (inside) {
         float k[24], v[3];
         gather_corners<S>(g, k);
