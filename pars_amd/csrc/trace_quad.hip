@@ -536,6 +536,10 @@ __device__ __forceinline__ void pair_finish(const PairGather& P, bool pair, bool
 // Pair gathers while more than this many lanes of the wave are live.
 constexpr int kPairLive = 32;
 
+__device__ __forceinline__ float4 pick4(bool c, float4 x, float4 y) {  // value select (no lvalue select)
+  return make_float4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
+}
+
 // Hybrid union trace into 8-seed grouped slots (PA_SLOTS_GROUP8: slot s of
 // seed p = vec4 at ((p / 8) * region + s) * 8 + p % 8; a wave's stores of one
 // slot fill whole 128-B lines, and a group's slots are contiguous).  Every lane starts with one seed
@@ -665,65 +669,62 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         a = finish_step<S, MODE, INTERP, STOP_RUNTIME>(cfg, f, last, v, next, rule);
       }
     }
-    if (st.state != Q_IDLE) {
-      float4 val = termination_vertex();
-      ++st.slot;
-      if (st.state == Q_START) {
-        val = last;
-        if (st.R < 2) {
-          st.state = Q_DONE;
+    // The segment machine of one iteration, as selects where it can be (a
+    // merge of many paths costs register copies in every iteration): START
+    // emits the seed vertex, STEP emits the stepped vertex, or the
+    // termination vertex when the segment ends (last iteration, stop rule,
+    // left the block); a particle that leaves the block may continue in a
+    // neighbour (START again) or as a continuation record.
+    {
+      const bool active = st.state != Q_IDLE;
+      const bool start = st.state == Q_START;
+      const bool stepping = active && !start && st.it != st.R - 1;
+      if constexpr (!kCells) {
+        if (stepping) a = advance_rt<S, MODE, INTERP>(cfg, f, last, next, rule);
+      }
+      const bool stepped = stepping && a == 0;
+      const float4 val = pick4(start, last, pick4(stepped, next, termination_vertex()));
+      int nstate = start ? (st.R < 2 ? Q_DONE : Q_STEP) : (stepped ? Q_STEP : Q_DONE);
+      st.it = start ? 1 : st.it + int(stepped);
+      st.nsteps += int(stepped);
+      last = pick4(stepped, next, last);
+      st.slot += int(active);
+      if (stepping && a == 1) {  // left the block (last is the point that did)
+        int nb = -1;
+        bool go;
+        if (st.b < 0) {
+          const int k = face_rule(f, last, cfg.local_mask);
+          go = k >= 0 && cfg.has_field[k + 1];
+          nb = k;
         } else {
-          st.state = Q_STEP;
-          st.it = 1;
+          go = face_rule(f, last, cfg.face_mask[st.b]) == (st.b ^ 1);
         }
-      } else if (st.it == st.R - 1) {
-        st.state = Q_DONE;
-      } else {
-        if constexpr (!kCells) a = advance_rt<S, MODE, INTERP>(cfg, f, last, next, rule);
-        if (a == 0) {
-          val = next;
-          last = next;
-          ++st.it;
-          ++st.nsteps;
-        } else {
-          st.state = Q_DONE;
-          if (a == 1) {
-            int nb = -1;
-            bool go;
-            if (st.b < 0) {
-              const int k = face_rule(f, last, cfg.local_mask);
-              go = k >= 0 && cfg.has_field[k + 1];
-              nb = k;
-            } else {
-              go = face_rule(f, last, cfg.face_mask[st.b]) == (st.b ^ 1);
-            }
-            if (go) {
-              const int Rn = st.R - st.it;
-              if (int64_t(st.slot) + Rn > region) {
-                write_continuation(cont + atomicAdd(n_cont, 1u), last, Rn, nb, st.orank, st.pid);
-              } else {
-                st.b = nb;
-                st.R = Rn;
-                f = cfg.field[st.b + 1];
-                // wait for the descriptor here (a rare path): otherwise its
-                // registers count as pending at the loop head of every
-                // iteration, whose wait would then take the vertex store too
-                __builtin_amdgcn_s_waitcnt(0x0F70);
-                st.state = Q_START;
-              }
-            }
+        if (go) {
+          const int Rn = st.R - st.it;
+          if (int64_t(st.slot) + Rn > region) {
+            write_continuation(cont + atomicAdd(n_cont, 1u), last, Rn, nb, st.orank, st.pid);
+          } else {
+            st.b = nb;
+            st.R = Rn;
+            f = cfg.field[st.b + 1];
+            // wait for the descriptor here (a rare path): otherwise its
+            // registers count as pending at the loop head of every
+            // iteration, whose wait would then take the vertex store too
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            nstate = Q_START;
           }
         }
       }
+      st.state = active ? nstate : Q_IDLE;
       if constexpr (kCells) {
-        pend = lbase + (st.slot - 1) * 128;
+        pend = active ? lbase + (st.slot - 1) * 128 : kDrop;
         pval = val;
       } else {
-        verts[group8_slot(st.pid, st.slot - 1, region)] = val;
+        if (active) verts[group8_slot(st.pid, st.slot - 1, region)] = val;
       }
       const bool isT = is_termination_vertex(val);
-      if (st.slot > 1 && !st.prevT && !isT) ++st.pairs;
-      st.prevT = isT;
+      st.pairs += int(active && st.slot > 1 && !st.prevT && !isT);
+      st.prevT = active ? isT : st.prevT;
       if (st.state == Q_DONE) {
         counts[st.pid] = st.slot;
         if (seed_steps) seed_steps[st.pid] = st.nsteps;
