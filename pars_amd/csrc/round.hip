@@ -42,9 +42,13 @@ __global__ __launch_bounds__(kThreads) void k_round_counts(const float4* __restr
   icount[p] = pairs[p] + x;
 }
 
+// A DPP row shift; a lane without a source reads 0 (bound control), which
+// needs no copy of the source first.  Every use below replaces those lanes
+// (j == 7 after row_shl:1, j == 0 after row_shr:1) or reads only lanes that
+// have a source (row_shr:7 at j == 7, row_shl:7 at j == 0).
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ float4 dpp_f4(float4 v) {
