@@ -35,9 +35,11 @@ constexpr int kBcast0 = 0x00, kBcast1 = 0x55, kBcast2 = 0xAA;
 constexpr int kSwap1 = 0xB1;  // (1,0,3,2)
 constexpr int kSwap2 = 0x4E;  // (2,3,0,1)
 
+// Every lane has a source under a quad permutation, so the bound control
+// (0 for a lane without one) never applies; it spares a copy of v.
 template <int CTRL>
 __device__ __forceinline__ int qmov(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ float qmovf(float v) {
