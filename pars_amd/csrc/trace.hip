@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const TraceConfig cfg, const p
         exited = a == 1;
         break;
       }
-      out[it] = next;
+      stream_store(&out[it], next);
       last = next;
       ++steps;
     }
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_union(const TraceConfig cfg, c
           exited = a == 1;
           break;
         }
-        out(base + it) = next;
+        stream_store(&out(base + it), next);
         last = next;
         ++steps;
       }

@@ -659,7 +659,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       const bool even = (lane & 1) == 0;
       if constexpr (storage_of(S) == PA_STORAGE_FP32) pair_issue(r, pair, even, P);
       else gather_record<S>(reinterpret_cast<const uint4*>(r), g);
-      store16(vrs, pval, pend);
+      store16<kStream>(vrs, pval, pend);
       pend = kDrop;
       // keep the exchange (which waits for the loads) after the store
       __builtin_amdgcn_sched_barrier(0);
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         pend = active ? lbase + (st.slot - 1) * 128 : kDrop;
         pval = val;
       } else {
-        if (active) verts[group8_slot(st.pid, st.slot - 1, region)] = val;
+        if (active) stream_store(&verts[group8_slot(st.pid, st.slot - 1, region)], val);
       }
       const bool isT = is_termination_vertex(val);
       st.pairs += int(active && st.slot > 1 && !st.prevT && !isT);
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     }
   }
 
-  if constexpr (kCells) store16(vrs, pval, pend);
+  if constexpr (kCells) store16<kStream>(vrs, pval, pend);
 
   // ---- quad mode: live particle k of the wave goes to quad k
   const int nlive = __popcll(live);
@@ -803,7 +803,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         const uint32_t ci = quad_sum(wl ? 0u : static_cast<uint32_t>(static_cast<int>(sc)) * L.stride);
         load_record<S>(L.data, c, ci, G.A, G.B);
       }
-      store4(vrs, __float_as_int(qval), qpend);
+      store4<kStream>(vrs, __float_as_int(qval), qpend);
       qpend = kDrop;
       if (qs.state != Q_IDLE) {
         const float val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont,
@@ -825,7 +825,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         }
       }
     }
-    store4(vrs, __float_as_int(qval), qpend);
+    store4<kStream>(vrs, __float_as_int(qval), qpend);
   }
   wave_steps = wave_sum_u64(wave_steps);
   if (lane == 0 && wave_steps) atomicAdd(steps_out, wave_steps);
