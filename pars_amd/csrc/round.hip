@@ -336,7 +336,8 @@ __global__ __launch_bounds__(kSplitThreads) void k_group8_split(const float4* __
                                                                 const int64_t* __restrict__ ioff, int64_t vcap,
                                                                 int64_t icap, float param,
                                                                 float4* __restrict__ vout, int32_t* __restrict__ iout,
-                                                                float4* __restrict__ cout, unsigned* flags) {
+                                                                float4* __restrict__ cout, unsigned* flags,
+                                                                unsigned* __restrict__ queue) {
   const int64_t NV = voff[n], NI = ioff[n];
   if (NV > vcap || NI > icap || NV > INT32_MAX) return;  // the same for every wave
   const int w = int(threadIdx.x >> 6), lane = int(threadIdx.x & 63);
@@ -351,11 +352,21 @@ __global__ __launch_bounds__(kSplitThreads) void k_group8_split(const float4* __
     // row); the colour waves only need to keep their rings from filling.
     // Issue priority to the packer: 0.89 -> 0.83 ms on the share round.
     __builtin_amdgcn_s_setprio(3);
-    const int64_t g = int64_t(blockIdx.x);
-    if (g * 8 < n) {
-      RingSink sink{rings, 0u, 0u, flags, true};
+    // Persistent workgroups: the grid is as many as fit on the device at
+    // once; workgroup b packs group b, then the groups that a queue counter
+    // (zeroed by k_round_totals) hands out.  A group costs ~4 us of
+    // dependent header loads besides 0.8 us per row, and a workgroup per
+    // group also left its slots to colour waves draining their rings: the
+    // share round's finish 0.82 -> 0.72 ms.  The next group is taken before
+    // the current one is packed, so its atomic is in flight meanwhile.
+    RingSink sink{rings, 0u, 0u, flags, true};
+    const int64_t groups = (n + 7) / 8;
+    for (int64_t g = int64_t(blockIdx.x); g < groups;) {
+      uint32_t next = 0;
+      if (lane == 0) next = atomicAdd(queue, 1u);
       int64_t O0 = 0;
       pack_group<true>(g, sink, slots, n, region, counts, cross, icount, voff, ioff, NI, vout, iout, O0);
+      g = int64_t(gridDim.x) + int64_t(__builtin_amdgcn_readfirstlane(next));
     }
     __hip_atomic_store(&rings[0].done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     __hip_atomic_store(&rings[1].done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -371,10 +382,11 @@ size_t finish_ws_bytes(int64_t n) {
 }
 
 __global__ void k_round_totals(const int64_t* __restrict__ voff, const int64_t* __restrict__ ioff, int64_t n,
-                               int64_t* __restrict__ totals) {
+                               int64_t* __restrict__ totals, unsigned* __restrict__ queue) {
   if (threadIdx.x == 0) {
     totals[0] = voff[n];
     totals[1] = ioff[n];
+    *queue = 0u;  // the finish's group queue (a kernel store: capturable)
   }
 }
 
@@ -390,23 +402,32 @@ pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_
   int32_t* cross = icount + n;
   char* tail = reinterpret_cast<char*>(cross + n);
   auto* sws = reinterpret_cast<int64_t*>(tail + (256 - (reinterpret_cast<uintptr_t>(tail) & 255)));
+  // the group queue sits past the scan workspace, inside the 4 KiB slack
+  auto* queue = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(sws) + (scan_workspace_bytes(n) + 255) / 256 * 256);
   const auto* slots = reinterpret_cast<const float4*>(d_slots);
   launch_begin(ctx, "round_counts");
   hipLaunchKernelGGL(k_round_counts, dim3(unsigned((n + kThreads - 1) / kThreads)), dim3(kThreads), 0, ctx->stream,
                      slots, n, region, d_counts, d_pairs, icount, cross);
   PA_TRY(scan_i32(ctx, d_counts, n, voff, sws));
   PA_TRY(scan_i32(ctx, icount, n, ioff, sws));
-  hipLaunchKernelGGL(k_round_totals, dim3(1), dim3(64), 0, ctx->stream, voff, ioff, n, totals);
+  hipLaunchKernelGGL(k_round_totals, dim3(1), dim3(64), 0, ctx->stream, voff, ioff, n, totals, queue);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   const int64_t groups = (n + 7) / 8;
+  int cus = 0;
+  PA_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
   launch_begin(ctx, "round_finish");
   if (!with_color_mode(mode, [&](auto M) {
-        hipLaunchKernelGGL((k_group8_split<decltype(M)::value>), dim3(unsigned(groups)), dim3(kSplitThreads), 0,
-                           ctx->stream, slots, n, region, d_counts, cross, icount, voff, ioff,
-                           int64_t(vertex_capacity), int64_t(index_capacity), param,
-                           reinterpret_cast<float4*>(d_vertices), d_indices, reinterpret_cast<float4*>(d_colors),
-                           flags);
+        constexpr int MD = decltype(M)::value;
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group8_split<MD>, kSplitThreads, 0) != hipSuccess ||
+            per_cu < 1)
+          per_cu = 1;
+        const int64_t grid = std::min<int64_t>(groups, int64_t(cus > 0 ? cus : 1) * per_cu);
+        hipLaunchKernelGGL(k_group8_split<MD>, dim3(unsigned(grid)), dim3(kSplitThreads), 0, ctx->stream, slots, n, region,
+                           d_counts, cross, icount, voff, ioff, int64_t(vertex_capacity), int64_t(index_capacity),
+                           param, reinterpret_cast<float4*>(d_vertices), d_indices,
+                           reinterpret_cast<float4*>(d_colors), flags, queue);
       }))
     return fail(ctx, PA_E_INVALID, "unknown color mode");
   launch_end(ctx);
