@@ -218,6 +218,42 @@ def test_trace_union_bitexact(ctx, orc, storage, region_extra, layout, engine, o
     assert (ou["segment_block"] >= 0).any()  # some seeds did cross into neighbours
 
 
+@pytest.mark.parametrize("n", [1, 13, 2049, 30001])
+def test_union_round_fused_sizes(ctx, orc, n):
+    """The fused finish runs as many workgroups as fit at once and hands the
+    remaining 8-seed groups out through a queue: fewer groups than
+    workgroups, a partial last group and many groups per workgroup all give
+    the separate stages' vertices, indices and colours, twice in a row (the
+    queue is reset by the round itself)."""
+    S = ShareFields(ctx, orc, 128, 64, 21, pa.FP32, layout=pa.LAYOUT_CELLS)
+    sh = S.share
+    dt, R = 0.02, 120
+    seeds = orc.seed_generate(sh.local.offset, sh.local.size, sh.local.spacing, R, 21)  # one seed per node
+    assert len(seeds) >= n
+    seeds = seeds[:n]
+    parts = pa.Particles.from_host(ctx, seeds)
+    region = R + 40
+    t = S.tracer(ctx, dt, pa.RUNGE_KUTTA_4, pa.INTERP_REFERENCE, pa.ENGINE_HYBRID)
+    t2 = S.tracer(ctx, dt, pa.RUNGE_KUTTA_4, pa.INTERP_REFERENCE, pa.ENGINE_HYBRID)  # owns u's buffers
+    u = t2.trace_union(parts, region, order="group8")
+    total = int(u["counts"].download(np.int32, n).sum())
+    packed, m, _ = pa.particle_tracer.prune_counted_group8(ctx, u["vertices"].ptr, region, u["counts"], n, total)
+    idx, ni = pa.index_generator.generate(ctx, packed, m)
+    # a vertex that the colour rule leaves alone keeps what the buffer held
+    sentinel = np.full(n * region * 4, 7.5, np.float32)  # the round's vertex capacity
+    col_sep = ctx.upload(sentinel)
+    ctx.check(pa.lib().pa_gpu_color_generate(ctx.handle, pa.C.c_void_p(packed.ptr), pa.C.c_size_t(m),
+                                             pa.C.c_void_p(idx.ptr), pa.C.c_size_t(ni), pa.C.c_int(2),
+                                             pa.C.c_float(0.6), pa.C.c_void_p(col_sep.ptr)), "color")
+    for _ in range(2):
+        r = t.trace_union_round(parts, region, 2, 0.6, colors=ctx.upload(sentinel))
+        assert r["continuation"].n == 0
+        assert r["n_vertices"] == m and r["n_indices"] == ni
+        assert r["vertices"].download(np.float32, 4 * m).tobytes() == packed.download(np.float32, 4 * m).tobytes()
+        assert np.array_equal(r["indices"].download(np.int32, ni), idx.download(np.int32, ni))
+        assert r["colors"].download(np.float32, 4 * m).tobytes() == col_sep.download(np.float32, 4 * m).tobytes()
+
+
 @pytest.mark.parametrize("storage,layout", [(pa.FP32, pa.LAYOUT_CELLS), (pa.BF16, pa.LAYOUT_NODES)])
 @pytest.mark.parametrize("mode", [2, 0, 4, 100])
 def test_union_round_fused(ctx, orc, storage, layout, mode):
