@@ -572,10 +572,11 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   __shared__ QBlock blk[7];
   __shared__ float4 x_last[kQuadBlock / 64][kQuadSwitch];
   __shared__ int32_t x_int[kQuadBlock / 64][kQuadSwitch][10];
-  // FP32 cell records: the lane's record of its previous step (chunk-major,
-  // so a wave's 16-B reads of one chunk fall in distinct banks)
-  constexpr bool kReuse = layout_of(S) == PA_LAYOUT_CELLS && storage_of(S) == PA_STORAGE_FP32;
-  __shared__ uint4 prev_rec[kReuse ? 6 : 1][kReuse ? kQuadBlock : 1];
+  // Cell records: the lane's record of its previous step (chunk-major, so
+  // a wave's 16-B reads of one chunk fall in distinct banks)
+  constexpr bool kReuse = layout_of(S) == PA_LAYOUT_CELLS;
+  constexpr int kChunks = kReuse ? rec_used<S>() : 1;
+  __shared__ uint4 prev_rec[kChunks][kReuse ? kQuadBlock : 1];
   fill_tables<S>(cfg, tab, blk);
   __syncthreads();
 
@@ -650,7 +651,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         m[i] = static_cast<int>(fminf(fmaxf(floorf(q[i]), 0.0f), static_cast<float>(f.dims[i] - 2)));
       uint64_t r = reinterpret_cast<uint64_t>(cell_record<S>(f, m));
       const uint64_t rown = r;
-      // While the wave is full (pair gathers), a particle still in the cell
+      // While the wave is full (more than kPairLive live lanes, where FP32
+      // records are pair-gathered), a particle still in the cell
       // of its previous step takes that record from LDS and gathers like a
       // lane that does not step (no line of its own): share round R = 52
       // 0.241 -> 0.219 ms; at R = 1002 no change (1.389 -> 1.391 ms), as
@@ -681,7 +683,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       if constexpr (storage_of(S) == PA_STORAGE_FP32) pair_finish<S>(P, pair, even, g);
       if (kReuse && __popcll(live) > kPairLive) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
+        for (int k = 0; k < kChunks; ++k) {
           g.r[k] = sel4(reuse, prev_rec[k][threadIdx.x], g.r[k]);
           prev_rec[k][threadIdx.x] = g.r[k];
         }
