@@ -187,30 +187,65 @@ def endpoint_errors(pa, orc, p, share, storage, interp, sample_stride):
                 p99_endpoint_err=float(np.percentile(d, 99)) if len(d) else None)
 
 
-def cpu_baseline(orc, share, sample_stride, threads, min_seconds=10.0):
-    """The oracle's FP32 restatement of the same union trace on host cores."""
-    from pars_amd import workloads as W
+def host_info(threads):
+    """Host CPU of this run: model, logical CPUs the OS reports and may
+    schedule this process on, and the threads the baseline used."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        import subprocess
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.strip())
+    except Exception:  # reported as unknown
+        nproc = None
+    return dict(cpu_model=model, nproc=nproc, os_cpu_count=os.cpu_count(),
+                affinity=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+                hardware_concurrency=None, omp_num_threads=os.environ.get("OMP_NUM_THREADS"), threads_used=threads)
 
-    blocks = [share.local] + list(share.neighbors)
-    f32 = [orc.Field(orc.abc_field_f64(b.ghosted, b.int_offset, share.h).astype(np.float32), b.offset, b.size,
-                     b.spacing) for b in blocks]
+
+def cpu_baseline(orc, share, oracle_fields, threads, gpu=None, min_seconds=10.0):
+    """The oracle's FP32 restatement of the same union trace (all 262,144
+    seeds, vertex output) on host cores, over the node values the device
+    holds (read back from the FP32 fields).  Its first pass is also the
+    parity check of the headline pass: per-seed slot counts, step count and
+    every vertex, byte for byte."""
     seeds = orc.seed_generate(share.local.offset, share.local.size, share.local.spacing * np.float32(2), R, 21)
-    sample = seeds[::sample_stride].copy()
     orc.set_threads(threads)
     steps, passes = 0, 0
+    parity = None
     t0 = time.perf_counter()
     while passes == 0 or time.perf_counter() - t0 < min_seconds:
-        u = orc.trace_union(sample, f32[0], f32[1:], share.neighbor_ranks, share.face_masks, dt=DT,
-                            integrator=orc.RK4, stop_rule=orc.STOP_NORM, stop_eps=1e-6,
-                            vertex_capacity=len(sample) * (R + EXTRA_SLOTS))
+        u = orc.trace_union(seeds, oracle_fields[0], oracle_fields[1:], share.neighbor_ranks, share.face_masks,
+                            dt=DT, integrator=orc.RK4, stop_rule=orc.STOP_NORM, stop_eps=1e-6,
+                            vertex_capacity=len(seeds) * (R + EXTRA_SLOTS))
+        if passes == 0 and gpu is not None:
+            t_check = time.perf_counter()
+            cnt = np.bincount(u["segment_seed"], weights=u["segment_length"], minlength=len(seeds)).astype(np.int64)
+            parity = dict(form=gpu["form"], seeds=len(seeds), steps_equal=bool(u["total_steps"] == gpu["steps"]),
+                          counts_equal=bool(np.array_equal(cnt, gpu["counts"].astype(np.int64))),
+                          vertices=int(len(u["vertices"])),
+                          vertices_bitexact=bool(len(u["vertices"]) == len(gpu["vertices"]) and
+                                                 u["vertices"].tobytes() == gpu["vertices"].tobytes()),
+                          seeds_bitexact=bool(seeds.tobytes() == gpu["seeds"].tobytes()))
+            parity["ok"] = all(parity[k] for k in ("steps_equal", "counts_equal", "vertices_bitexact",
+                                                   "seeds_bitexact"))
+            t0 += time.perf_counter() - t_check  # the comparison is not CPU-baseline time
         steps += u["total_steps"]
         passes += 1
+        del u
     dt = time.perf_counter() - t0
+    hi = host_info(threads)
+    hi["hardware_concurrency"] = orc.hardware_concurrency()
     return dict(value=steps / dt, unit="particle-steps/s", cores=threads, kind="port",
-                sample="%d passes of the union trace with vertex output over %s of the 262144 seeds "
-                       "(%d seeds, %d steps per pass), %.1f s" %
-                       (passes, "all" if sample_stride == 1 else "every %dth" % sample_stride, len(sample),
-                        u["total_steps"], dt))
+                sample="%d passes of the union trace with vertex output over all %d seeds (%d steps per pass), "
+                       "%.1f s, %d threads" % (passes, len(seeds), steps // passes, dt, threads),
+                host=hi), parity
 
 
 def kernel_name(form):
@@ -425,6 +460,7 @@ def main():
     ap.add_argument("--no-workloads", action="store_true", help="skip the FTLE, batched and serving workloads")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
+    # one GPU's share of the host is 16 CPUs on the boxes this runs on
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -523,10 +559,22 @@ def main():
             t0 = time.perf_counter()
             extra[form]["fp64_error"] = endpoint_errors(pa, orc, p, share, st, interp, 32)
             log("errors %s (%.1f s): %s" % (form, time.perf_counter() - t0, extra[form]["fp64_error"]))
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(orc, share, 1, args.cpu_threads)
-        log("cpu baseline:", cpu)
+        # the headline pass once more, kept for the parity check against the oracle
+        p0 = pipes[forms[0]][0]
+        info = p0.step()
+        n0 = info["particles"]
+        gpu = dict(form=forms[0], steps=info["steps"], counts=p0.ub["counts"].download(np.int32, n0),
+                   vertices=p0.pruned.download(np.float32, 4 * info["vertices"]).reshape(-1, 4),
+                   seeds=p0.seeds.download(pa.PARTICLE_DTYPE, n0))
+        src = fields[(st0, layout_of_form(forms[0]))]
+        ofields = [orc.Field(f.read(), b.offset, b.size, b.spacing) for f, b in zip(src, blocks)]
+        t0 = time.perf_counter()
+        cpu, parity = cpu_baseline(orc, share, ofields, args.cpu_threads, gpu)
+        del gpu, ofields
+        log("cpu baseline (%.1f s): %s" % (time.perf_counter() - t0, cpu))
+        log("parity of the headline pass against the oracle:", parity)
 
     workloads = {}
     if not args.no_workloads:
@@ -552,7 +600,7 @@ def main():
                     seeds=n_seeds, remaining_iterations=R, dt=DT, field="ABC A=sqrt3 B=sqrt2 C=1",
                     form=forms[0], device=dev["name"]),
         pass_latency_ms=base["pass_latency_ms"],
-        roofline=roofline, cpu_baseline=cpu, forms=extra, workloads=workloads,
+        roofline=roofline, cpu_baseline=cpu, parity=parity, forms=extra, workloads=workloads,
         bf16_vs_fp32=bf16_ratio(forms, extra),
     )
     print(json.dumps(line), flush=True)

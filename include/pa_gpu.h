@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PA_GPU_ABI_VERSION 1
+#define PA_GPU_ABI_VERSION 2
 
 typedef enum pa_status {
   PA_OK = 0,
@@ -357,7 +357,7 @@ pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slot
                                             const int32_t* d_vertex_counts, float* d_out, size_t capacity,
                                             size_t* n_out, int64_t* d_offsets);
 /* Counted prune of 8-seed grouped slots (PA_SLOTS_GROUP8) into the pruned
- * layout. */
+ * layout; region_slots below 2^24 (PA_E_INVALID otherwise). */
 pa_status   pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
                                         const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,
                                         size_t* n_out, int64_t* d_offsets);
@@ -386,16 +386,22 @@ typedef struct pa_pool {
   int32_t* field;        /* field index */
   int32_t* request;      /* optional: request id (the particle's original_rank) */
   int32_t* request_done; /* optional: finished particles per request id (zeroed by the caller) */
+  int32_t  n_requests;   /* entries of request_done; request ids are in [0, n_requests) */
 } pa_pool;
 /* Fill pool entries [0, n) from particles; entries whose field index is not
- * in [0, n_fields) become PA_POOL_INVALID and the call returns PA_E_INVALID
- * after initialising the rest. */
+ * in [0, n_fields), or whose request id (original_rank) is not in
+ * [0, n_requests) when request is given, become PA_POOL_INVALID and the call
+ * returns PA_E_INVALID after initialising the rest.  request needs
+ * request_done and n_requests >= 1. */
 pa_status   pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
                              const pa_pool* pool);
 /* Advance every active entry of [0, n) by at most max_iterations reference
  * iterations (0 = to the end) with the tracer's integrator, step size, stop
  * rule and interpolation; waves_per_cu = resident waves of the persistent
- * kernel per CU (0 = 16).  All fields share one storage and layout. */
+ * kernel per CU (0 = 16).  All fields share one storage and layout.  An
+ * active entry whose field index is n_fields or more becomes
+ * PA_POOL_INVALID without being stepped and the call returns PA_E_INVALID
+ * once the others have run. */
 pa_status   pa_gpu_pool_run(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
                             const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
                             uint64_t* total_steps);

@@ -35,6 +35,7 @@ HANDOFF_DTYPE = np.dtype(
 assert HANDOFF_DTYPE.itemsize == 64
 
 EULER, RK4, CLASSIC_RK4 = 0, 2, 100
+RK4_DT_OVER_6 = 102  # runge_kutta4_classic's coefficient rounding (risk measurement only)
 STOP_ISZERO, STOP_NORM = 0, 1
 INTERP_REFERENCE, INTERP_CONSISTENT = 0, 1
 TERMINATION = np.array([-2.0, -2.0, -2.0, 0.0], np.float32)
@@ -124,6 +125,10 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib().orc_get_threads())
+
+
+def hardware_concurrency() -> int:
+    return int(lib().orc_hardware_concurrency())
 
 
 class Field:

@@ -153,10 +153,28 @@ bool stops(const T v[3], int rule, T eps) {
 }
 
 // Frozen-derivative step (particle_tracer.cpp:315-324): the odeint system
-// returns the same vector for every stage, so runge_kutta4 (Boost.Odeint
-// 1.69 explicit_generic_rk, final stage scale_sum5(1, b0*dt, b1*dt, b2*dt,
-// b3*dt) evaluated left to right) becomes a five-term sum; euler is
-// scale_sum2(1, dt).  d = (v0, v1, v2, 0).
+// returns the same vector for every stage.  Boost.Odeint 1.69 is not
+// vendored in the reference (a third-party dependency, Boost 1.69.0 per
+// utility/shell/rwth/load_common_modules.zsh:5); its published algorithm:
+//  - pa::runge_kutta_4_integrator is odeint::runge_kutta4<Vector4f, float,
+//    Vector4f, float, vector_space_algebra> (integrators.hpp:12-17), i.e.
+//    explicit_generic_rk<4, 4, ...> with the Butcher tableau of
+//    boost/numeric/odeint/stepper/runge_kutta4.hpp: b = {1/6, 1/3, 1/3,
+//    1/6}, each entry static_cast<Value>(1) / static_cast<Value>(6 or 3)
+//    with Value = float;
+//  - the last stage of detail/generic_rk_algorithm.hpp (calculate_stage,
+//    stage_number == StageCount) calls the algebra with
+//    generic_rk_scale_sum<4>(b, dt) (detail/generic_rk_operations.hpp),
+//    which is default_operations::scale_sum5(1.0, b[0]*dt, b[1]*dt,
+//    b[2]*dt, b[3]*dt): each coefficient is the float product b_k * dt;
+//  - vector_space_algebra::for_each6 applies the operation to the whole
+//    Eigen vectors: t1 = a1*t2 + a2*t3 + a3*t4 + a4*t5 + a5*t6, one Eigen
+//    expression evaluated per component left to right, each product rounded
+//    before its sum; pa is compiled with -msse -msse2 and no FMA
+//    (pa/CMakeLists.txt:37-38), so nothing is contracted.
+// With all four stage derivatives equal to d = (v0, v1, v2, 0) this is the
+// five-term sum below.  euler is scale_sum2(1, dt).  runge_kutta4_classic
+// (not used by pa) would round dt/6 and dt/3 instead: ORC_RK4_DT_OVER_6.
 template <class T>
 void step_frozen(int integrator, const T p[4], const T v[3], T dt, T out[4]) {
   const T d[4] = {v[0], v[1], v[2], T(0)};
@@ -164,8 +182,13 @@ void step_frozen(int integrator, const T p[4], const T v[3], T dt, T out[4]) {
     for (int c = 0; c < 4; ++c) out[c] = T(1) * p[c] + dt * d[c];
     return;
   }
-  const T b0 = T(1) / T(6), b1 = T(1) / T(3), b2 = T(1) / T(3), b3 = T(1) / T(6);
-  const T a0 = b0 * dt, a1 = b1 * dt, a2 = b2 * dt, a3 = b3 * dt;
+  T a0, a1, a2, a3;
+  if (integrator == ORC_RK4_DT_OVER_6) {
+    a0 = dt / T(6), a1 = dt / T(3), a2 = dt / T(3), a3 = dt / T(6);
+  } else {
+    const T b0 = T(1) / T(6), b1 = T(1) / T(3), b2 = T(1) / T(3), b3 = T(1) / T(6);
+    a0 = b0 * dt, a1 = b1 * dt, a2 = b2 * dt, a3 = b3 * dt;
+  }
   for (int c = 0; c < 4; ++c)
     out[c] = (((T(1) * p[c] + a0 * d[c]) + a1 * d[c]) + a2 * d[c]) + a3 * d[c];
 }
@@ -355,6 +378,7 @@ extern "C" {
 
 void orc_set_threads(int n) { g_threads = n; }
 int orc_get_threads(void) { return thread_count(); }
+int orc_hardware_concurrency(void) { return int(std::thread::hardware_concurrency()); }
 
 // seed_generator.cpp:9-21 with unravel_index (index.hpp:9-19): count per
 // axis = int(size / stride) (FP32 divide, truncation); position =

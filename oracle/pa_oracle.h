@@ -78,8 +78,13 @@ typedef struct orc_field64 {
 /* Integrators, in the order of pa::variant_integrator
  * (pa/include/pa/math/integrators.hpp:25-33).  Only euler and
  * runge_kutta_4 are restated; ORC_CLASSIC_RK4 is the textbook four-gather
- * RK4 (a build extension without a reference counterpart). */
-enum { ORC_EULER = 0, ORC_RK4 = 2, ORC_CLASSIC_RK4 = 100 };
+ * RK4 (a build extension without a reference counterpart).
+ * ORC_RK4_DT_OVER_6 is not a reference integrator: it is the frozen step
+ * with the coefficients Boost.Odeint's runge_kutta4_classic would use
+ * (dt/6, dt/3, dt/3, dt/6 rounded once each) instead of runge_kutta4's
+ * (1/6)*dt etc., used only by the tests that measure how much the unpinned
+ * rounding of the final stage could move a trajectory. */
+enum { ORC_EULER = 0, ORC_RK4 = 2, ORC_CLASSIC_RK4 = 100, ORC_RK4_DT_OVER_6 = 102 };
 
 /* Stop rules: the reference's Eigen isZero() test (every |v_i| <= 1e-5f,
  * particle_tracer.cpp:312) or the extension |v| < eps. */
@@ -126,6 +131,8 @@ typedef struct orc_handoff {
 
 void     orc_set_threads(int n);
 int      orc_get_threads(void);
+/* std::thread::hardware_concurrency() of this host (for reports). */
+int      orc_hardware_concurrency(void);
 
 /* seed_generator.cpp:9-21 */
 int64_t  orc_seed_count(const float size[3], const float stride[3], int32_t dims_out[3]);

@@ -568,7 +568,7 @@ POOL_ACTIVE, POOL_DONE, POOL_INVALID = 0, 1, 2
 
 class _PoolC(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in ("position", "iteration", "remaining", "steps", "status", "field",
-                                                 "request", "request_done")]
+                                                 "request", "request_done")] + [("n_requests", C.c_int32)]
 
 
 class ParticlePool:
@@ -584,8 +584,9 @@ class ParticlePool:
         self.steps = ctx.alloc(c * 4)
         self.status = ctx.alloc(c * 4)
         self.field = ctx.alloc(c * 4)
+        self.requests = int(requests)
         self.request = ctx.alloc(c * 4) if requests else None
-        self.request_done = ctx.upload(np.zeros(max(requests, 1), np.int32)) if requests else None
+        self.request_done = ctx.upload(np.zeros(requests, np.int32)) if requests else None
         self.status.upload(np.full(c, POOL_INVALID, np.int32))  # entries never initialised are skipped
         self.n = 0
 
@@ -594,16 +595,17 @@ class ParticlePool:
             return C.c_void_p(b.ptr + offset * size) if b is not None else None
         return _PoolC(at(self.position, 16), at(self.iteration, 4), at(self.remaining, 4), at(self.steps, 4),
                       at(self.status, 4), at(self.field, 4), at(self.request, 4),
-                      C.c_void_p(self.request_done.ptr) if self.request_done is not None else None)
+                      C.c_void_p(self.request_done.ptr) if self.request_done is not None else None, self.requests)
 
     def init(self, particles: "Particles", n_fields: int, offset: int = 0):
         """Entries [offset, offset + particles.n) from particles (field = vector_field_index,
         request = original_rank)."""
         assert offset + particles.n <= self.capacity
         pc = self._c(offset)
-        self.ctx.check(lib().pa_gpu_pool_init(self.ctx.handle, C.c_void_p(particles.ptr), C.c_size_t(particles.n),
-                                              C.c_int32(n_fields), C.byref(pc)), "pool_init")
-        self.n = max(self.n, offset + particles.n)
+        st = lib().pa_gpu_pool_init(self.ctx.handle, C.c_void_p(particles.ptr), C.c_size_t(particles.n),
+                                    C.c_int32(n_fields), C.byref(pc))
+        self.n = max(self.n, offset + particles.n)  # rejected entries are initialised as invalid
+        self.ctx.check(st, "pool_init")
 
     def run(self, tracer: "particle_tracer", fields, n: int = None, max_iterations: int = 0, waves_per_cu: int = 0):
         arr = (C.c_void_p * len(fields))(*[f.handle for f in fields])

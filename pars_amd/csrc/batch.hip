@@ -8,6 +8,7 @@
 // step count are those of a one-block reference trace (the vertices are not
 // kept).  A particle's result does not depend on what else is in the pool.
 #include <cstring>
+#include <string>
 
 #include "pa_device.hpp"
 
@@ -27,7 +28,7 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_init(const pa_particle* __r
                                                           float4* __restrict__ pos, int32_t* __restrict__ it,
                                                           int32_t* __restrict__ rem, int32_t* __restrict__ steps,
                                                           int32_t* __restrict__ status, int32_t* __restrict__ field,
-                                                          int32_t* __restrict__ request,
+                                                          int32_t* __restrict__ request, int n_requests,
                                                           unsigned int* __restrict__ invalid) {
   const int64_t i = int64_t(blockIdx.x) * kPoolBlock + threadIdx.x;
   if (i >= n) return;
@@ -36,10 +37,15 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_init(const pa_particle* __r
   it[i] = 1;
   rem[i] = p.remaining_iterations;
   steps[i] = 0;
-  const bool ok = p.vector_field_index >= 0 && p.vector_field_index < n_fields;
+  bool ok = p.vector_field_index >= 0 && p.vector_field_index < n_fields;
+  if (request) {
+    // the request id indexes request_done: an id outside it is never stored
+    const bool known = p.original_rank >= 0 && p.original_rank < n_requests;
+    request[i] = known ? p.original_rank : 0;
+    ok = ok && known;
+  }
   status[i] = ok ? PA_POOL_ACTIVE : PA_POOL_INVALID;
   field[i] = ok ? p.vector_field_index : 0;
-  if (request) request[i] = p.original_rank;
   if (!ok) atomicAdd(invalid, 1u);
 }
 
@@ -54,9 +60,11 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
                                                          int32_t* __restrict__ status_a,
                                                          const int32_t* __restrict__ field_a,
                                                          const int32_t* __restrict__ request,
-                                                         int32_t* __restrict__ request_done, int64_t n, int budget,
-                                                         int rule, unsigned long long* __restrict__ work,
-                                                         unsigned long long* __restrict__ steps_out) {
+                                                         int32_t* __restrict__ request_done, int n_requests,
+                                                         int64_t n, int budget, int rule,
+                                                         unsigned long long* __restrict__ work,
+                                                         unsigned long long* __restrict__ steps_out,
+                                                         unsigned long long* __restrict__ rejected) {
   const int lane = threadIdx.x & 63;
   bool have = false, open = true;
   int64_t i = 0;
@@ -77,14 +85,21 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
       if (!have) {
         const unsigned long long idx = base + uint32_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
         if (idx < static_cast<unsigned long long>(n) && status_a[idx] == PA_POOL_ACTIVE) {
-          i = int64_t(idx);
-          p = pos[i];
-          it = it_a[i];
-          R = rem[i];
-          st = steps_a[i];
-          f = cfg.f[field_a[i]];
-          left = budget;
-          have = true;
+          const int32_t fi = field_a[idx];
+          if (fi < 0 || fi >= cfg.n_fields) {
+            // a field this launch was not given: never stepped, never read
+            status_a[idx] = PA_POOL_INVALID;
+            atomicAdd(rejected, 1ull);
+          } else {
+            i = int64_t(idx);
+            p = pos[i];
+            it = it_a[i];
+            R = rem[i];
+            st = steps_a[i];
+            f = cfg.f[fi];
+            left = budget;
+            have = true;
+          }
         }
       }
     }
@@ -111,7 +126,10 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
         steps_a[i] = st;
         if (fin) {
           status_a[i] = PA_POOL_DONE;
-          if (request && request_done) atomicAdd(&request_done[request[i]], 1);
+          if (request && request_done) {
+            const int32_t r = request[i];
+            if (r >= 0 && r < n_requests) atomicAdd(&request_done[r], 1);
+          }
         }
         have = false;
       }
@@ -122,7 +140,7 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
 }
 
 __global__ void k_pool_zero(unsigned long long* c) {
-  if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+  if (threadIdx.x < 3) c[threadIdx.x] = 0ull;
 }
 
 }  // namespace
@@ -140,19 +158,24 @@ pa_status pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size
       !pool->field)
     return PA_E_INVALID;
   if (n_fields < 1 || n_fields > kMaxFields) return fail(ctx, PA_E_INVALID, "pool: 1 to 16 fields");
+  if (pool->request && (!pool->request_done || pool->n_requests < 1))
+    return fail(ctx, PA_E_INVALID, "pool: request ids need request_done and n_requests >= 1");
   auto* invalid = reinterpret_cast<unsigned int*>(static_cast<char*>(ctx->counters) + 2080);
   hipLaunchKernelGGL(k_pool_zero, dim3(1), dim3(64), 0, ctx->stream,
                      reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2080));
   launch_begin(ctx, "pool_init");
   hipLaunchKernelGGL(k_pool_init, dim3(unsigned((n + kPoolBlock - 1) / kPoolBlock)), dim3(kPoolBlock), 0, ctx->stream,
                      d_particles, int64_t(n), int(n_fields), reinterpret_cast<float4*>(pool->position),
-                     pool->iteration, pool->remaining, pool->steps, pool->status, pool->field, pool->request, invalid);
+                     pool->iteration, pool->remaining, pool->steps, pool->status, pool->field, pool->request,
+                     int(pool->n_requests), invalid);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   unsigned bad = 0;
   PA_HIP(ctx, hipMemcpyAsync(&bad, invalid, sizeof bad, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (bad) return fail(ctx, PA_E_INVALID, "pool: particles with a field index outside [0, n_fields)");
+  if (bad)
+    return fail(ctx, PA_E_INVALID,
+                "pool: particles with a field index outside [0, n_fields) or a request id outside [0, n_requests)");
   return PA_OK;
 }
 
@@ -164,6 +187,8 @@ pa_status pa_gpu_pool_run(pa_gpu_tracer* t, const pa_gpu_field* const* fields, i
   if (n == 0) return PA_OK;
   if (!fields || n_fields < 1 || n_fields > kMaxFields) return fail(ctx, PA_E_INVALID, "pool: 1 to 16 fields");
   if (max_iterations < 0 || waves_per_cu < 0) return fail(ctx, PA_E_INVALID, "pool: negative budget or waves");
+  if (pool->request && pool->request_done && pool->n_requests < 1)
+    return fail(ctx, PA_E_INVALID, "pool: request ids need n_requests >= 1");
   PA_TRY(check_integrator_of(ctx, t->integrator));
   PoolConfig cfg;
   std::memset(&cfg, 0, sizeof cfg);
@@ -194,15 +219,18 @@ pa_status pa_gpu_pool_run(pa_gpu_tracer* t, const pa_gpu_field* const* fields, i
     hipLaunchKernelGGL((k_pool_run<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
                        dim3(unsigned(blocks)), dim3(kPoolBlock), 0, ctx->stream, cfg,
                        reinterpret_cast<float4*>(pool->position), pool->iteration, pool->remaining, pool->steps,
-                       pool->status, pool->field, pool->request, pool->request_done, int64_t(n), budget,
-                       t->stop_rule, counters, counters + 1);
+                       pool->status, pool->field, pool->request, pool->request_done, int(pool->n_requests),
+                       int64_t(n), budget, t->stop_rule, counters, counters + 1, counters + 2);
   }, cfg.f[0].layout);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
-  unsigned long long s = 0;
-  PA_HIP(ctx, hipMemcpyAsync(&s, counters + 1, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+  unsigned long long s[2] = {0, 0};
+  PA_HIP(ctx, hipMemcpyAsync(s, counters + 1, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (total_steps) *total_steps = s;
+  if (total_steps) *total_steps = s[0];
+  if (s[1])
+    return fail(ctx, PA_E_INVALID,
+                "pool: " + std::to_string(s[1]) + " entries name a field index >= n_fields; marked invalid");
   return PA_OK;
 }
 

@@ -59,9 +59,11 @@ __device__ __forceinline__ float4 dpp_f4(float4 v) {
 // starts at the group's first output element: a lane with nothing to store
 // gives the offset kDrop and the hardware drops the store, so every store is
 // issued by the whole wave without branches (exact wait counts) and without
-// traffic of its own.  Offsets are below 2^31 bytes (at most 8 seeds of at
-// most 2^24 slots each).
-constexpr int64_t kMaxRegion = int64_t(1) << 24;
+// traffic of its own.  A group's vertex offsets reach (8 * region - 1) * 16
+// bytes, which must stay below kDrop = 2^31 - 16 (an offset equal to kDrop
+// is dropped like the lanes that store nothing), so region is below 2^24.
+constexpr int64_t kMaxRegion = (int64_t(1) << 24) - 1;
+static_assert((8 * kMaxRegion - 1) * 16 < kDrop, "largest group-relative vertex offset must stay below kDrop");
 
 // What one lane of a packing wave knows about its seed p (lane 8k + j of the wave
 // holds slot s0 + j of seed k of the group).
@@ -394,7 +396,7 @@ pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_
                         const int32_t* d_pairs, int mode, float param, float* d_vertices, size_t vertex_capacity,
                         int32_t* d_indices, size_t index_capacity, float* d_colors, void* ws, int64_t* totals,
                         unsigned* flags) {
-  if (region > kMaxRegion) return fail(ctx, PA_E_INVALID, "round: at most 2^24 slots per seed");
+  if (region > kMaxRegion) return fail(ctx, PA_E_INVALID, "round: region_slots below 2^24");
   char* w = static_cast<char*>(ws);
   auto* voff = reinterpret_cast<int64_t*>(w);
   auto* ioff = voff + (n + 1);
@@ -447,7 +449,7 @@ extern "C" pa_status pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d
     *n_out = 0;
     return PA_OK;
   }
-  if (region_slots > size_t(kMaxRegion)) return fail(ctx, PA_E_INVALID, "prune_counted: at most 2^24 slots per seed");
+  if (region_slots > size_t(kMaxRegion)) return fail(ctx, PA_E_INVALID, "prune_counted: region_slots below 2^24");
   void* ws = nullptr;
   const size_t own = d_offsets ? 0 : (n + 1) * sizeof(int64_t);
   PA_TRY(scratch(ctx, own + scan_workspace_bytes(int64_t(n)) + 4096, &ws));

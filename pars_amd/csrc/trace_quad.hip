@@ -556,6 +556,15 @@ constexpr int kQuadSwitch = 16;
 // 1024 * region - 4 (the w component of seed 63's last slot), which with
 // region < 2^21 stay below kDrop = 2^31 - 16.
 constexpr int64_t kHybridMaxRegion = (int64_t(1) << 21) - 1;
+// Largest byte offset past the wave's first slot that a vertex store of the
+// hybrid engine can reach, plus the 16 bytes it stores: lane 63 (group 7,
+// seed 7) at slot region - 1.  Every store must end at or below kDrop, the
+// descriptor's range (an offset of kDrop is the dropped-store marker).
+__host__ __device__ constexpr int64_t hybrid_store_end(int64_t region) {
+  return ((7 * region + (region - 1)) * 8 + 7) * 16 + 16;
+}
+static_assert(hybrid_store_end(kHybridMaxRegion) <= kDrop, "hybrid vertex stores must stay inside the descriptor");
+static_assert(hybrid_store_end(kHybridMaxRegion + 1) > kDrop, "kHybridMaxRegion is the tight bound");
 
 template <int S, int MODE, int INTERP>
 __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
@@ -861,7 +870,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
 pa_status launch_hybrid_union(pa_gpu_ctx* ctx, const TraceConfig& cfg, int mode, const CoopLaunch& l,
                               unsigned long long* steps, unsigned* ncont, int32_t* pairs) {
   if (l.n > INT32_MAX) return fail(ctx, PA_E_INVALID, "hybrid engine: at most 2^31-1 seeds per call");
-  if (l.region > kHybridMaxRegion) return fail(ctx, PA_E_INVALID, "hybrid engine: region_slots below 2^21");
+  if (l.region > kHybridMaxRegion || l.region < 1 || hybrid_store_end(l.region) > kDrop)
+    return fail(ctx, PA_E_INVALID, "hybrid engine: region_slots below 2^21");
   const int64_t blocks = (l.n + kQuadBlock - 1) / kQuadBlock;
   dispatch(cfg.field[0].storage, mode, cfg.interp, [&](auto S, auto M, auto I) {
     hipLaunchKernelGGL((k_hybrid_union<decltype(S)::value, decltype(M)::value, decltype(I)::value>),

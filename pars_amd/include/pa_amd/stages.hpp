@@ -282,10 +282,12 @@ class particle_pool {
  public:
   particle_pool(context& c, size_t capacity, size_t requests = 0)
       : c_(&c), cap_(capacity), position_(c, capacity), iteration_(c, capacity), remaining_(c, capacity),
-        steps_(c, capacity), status_(c, capacity), field_(c, capacity), request_(c, capacity),
-        request_done_(c, std::vector<int32_t>(requests ? requests : 1, 0)) {
+        steps_(c, capacity), status_(c, capacity), field_(c, capacity), request_(c, requests ? capacity : 0),
+        request_done_(c, std::vector<int32_t>(requests, 0)) {
+    // without request counters the pool reads and writes no request id
     pool_ = pa_pool{reinterpret_cast<float*>(position_.data()), iteration_.data(), remaining_.data(), steps_.data(),
-                    status_.data(), field_.data(), request_.data(), request_done_.data()};
+                    status_.data(), field_.data(), requests ? request_.data() : nullptr,
+                    requests ? request_done_.data() : nullptr, int32_t(requests)};
   }
   // Load particles (vector_field_index selects the field, original_rank the
   // request).

@@ -37,12 +37,16 @@ def test_struct_layouts():
     assert pars_amd.PARTICLE_DTYPE.itemsize == 48  # pa::particle with Vector4f alignment
     assert C.sizeof(pars_amd.RoundInfo) == 40
     assert C.sizeof(pars_amd._Handoff) == 8 + 8 + 8 + 12 * 8
+    assert C.sizeof(pars_amd._PoolC) == 8 * 8 + 8  # eight pointers and n_requests (padded)
+    assert pars_amd._PoolC.n_requests.offset == 64
 
 
 def test_header_compiles_as_c_and_cpp(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "pa_gpu.h"\n_Static_assert(sizeof(pa_particle) == 48, "particle");\n'
-                   "_Static_assert(sizeof(pa_round_info) == 40, \"round\");\nint main(void){return 0;}\n")
+    src.write_text('#include <stddef.h>\n#include "pa_gpu.h"\n_Static_assert(sizeof(pa_particle) == 48, "particle");\n'
+                   "_Static_assert(sizeof(pa_round_info) == 40, \"round\");\n"
+                   "_Static_assert(sizeof(pa_pool) == 72 && offsetof(pa_pool, n_requests) == 64, \"pool\");\n"
+                   "int main(void){return 0;}\n")
     subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", str(src),
                     "-o", str(tmp_path / "t.o")], check=True)
     cpp = tmp_path / "t.cpp"

@@ -139,7 +139,7 @@ def test_trace_share_bitexact(ctx, orc, storage, integ, interp, layout, engine):
     assert np.array_equal(res["handoff_sources"], ohand["source"])
 
 
-@pytest.mark.parametrize("storage", [pa.FP32, pa.BF16])
+@pytest.mark.parametrize("storage", STORAGES)
 @pytest.mark.parametrize("region_extra", [64, 1])
 @pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
 @pytest.mark.parametrize("engine,order", [(pa.ENGINE_COOPERATIVE, "seed"), (pa.ENGINE_LANE, "seed"),
@@ -254,7 +254,8 @@ def test_union_round_fused_sizes(ctx, orc, n):
         assert r["colors"].download(np.float32, 4 * m).tobytes() == col_sep.download(np.float32, 4 * m).tobytes()
 
 
-@pytest.mark.parametrize("storage,layout", [(pa.FP32, pa.LAYOUT_CELLS), (pa.BF16, pa.LAYOUT_NODES)])
+@pytest.mark.parametrize("storage,layout", [(pa.FP32, pa.LAYOUT_CELLS), (pa.BF16, pa.LAYOUT_NODES),
+                                            (pa.BF16, pa.LAYOUT_CELLS), (pa.FP16, pa.LAYOUT_CELLS)])
 @pytest.mark.parametrize("mode", [2, 0, 4, 100])
 def test_union_round_fused(ctx, orc, storage, layout, mode):
     """The fused round (union trace + prune + index + colour in one call)
@@ -593,6 +594,51 @@ def test_pool_rejects_bad_field_index(ctx, orc):
     pool = pa.ParticlePool(ctx, len(parts))
     with pytest.raises(pa.PaError):
         pool.init(pa.Particles.from_host(ctx, parts), 2)
+
+
+def test_pool_request_ids_checked(ctx, orc):
+    """Request ids index request_done: without request counters the pool
+    reads no id (original_rank 3 is ignored and nothing is counted); with 2
+    counters an id of 3 or -1 is rejected and marks only its entry invalid;
+    a run given fewer fields than an entry names rejects that entry without
+    stepping it and still finishes the others."""
+    fields, fo = _pool_fields(ctx, orc, pa.FP32, pa.LAYOUT_NODES)
+    t = pa.particle_tracer(ctx)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.01)
+    parts = _pool_particles(orc, fo, 40, 9)
+    parts["original_rank"] = 3
+    n = len(parts)
+    dev = pa.Particles.from_host(ctx, parts)
+    plain = pa.ParticlePool(ctx, n)
+    plain.init(dev, 2)
+    total = plain.run(t, fields)
+    assert (plain.results()["status"] == pa.POOL_DONE).all() and total > 0
+    counted = pa.ParticlePool(ctx, n, requests=2)
+    for bad in (3, -1):
+        q = parts.copy()
+        q["original_rank"] = 1
+        q["original_rank"][[5, 17]] = bad
+        with pytest.raises(pa.PaError) as e:
+            counted.init(pa.Particles.from_host(ctx, q), 2)
+        assert e.value.status == pa.E_INVALID
+        st = counted.status.download(np.int32, n)
+        assert st[5] == st[17] == pa.POOL_INVALID and (np.delete(st, [5, 17]) == pa.POOL_ACTIVE).all()
+        counted.request_done.upload(np.zeros(2, np.int32))
+        counted.run(t, fields)
+        done = counted.request_done.download(np.int32, 2)
+        assert done[0] == 0 and done[1] == n - 2
+    # a run that is given only field 0: entries on field 1 are rejected, the rest finish
+    one = pa.ParticlePool(ctx, n)
+    one.init(dev, 2)
+    with pytest.raises(pa.PaError) as e:
+        one.run(t, fields[:1])
+    assert e.value.status == pa.E_INVALID
+    r = one.results()
+    on1 = parts["vector_field_index"] == 1
+    assert (r["status"][on1] == pa.POOL_INVALID).all() and (r["status"][~on1] == pa.POOL_DONE).all()
+    assert (r["steps"][on1] == 0).all()
+    assert r["position"][~on1].tobytes() == plain.results()["position"][~on1].tobytes()
 
 
 def test_serving_alone_equals_mix(ctx, orc):
