@@ -351,6 +351,24 @@ pa_status   pa_gpu_tracer_trace_union_round(pa_gpu_tracer* tracer, const pa_part
                                             size_t vertex_capacity, size_t* n_vertices, int32_t* d_indices,
                                             size_t index_capacity, size_t* n_indices, float* d_colors,
                                             float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps);
+/* No counterpart in the reference: the fused round above for callers that
+ * cannot allocate device memory while they run (code under stream capture,
+ * tight serving loops).  Same arguments and results as
+ * pa_gpu_tracer_trace_union_round; its workspace (index pairs, the
+ * finish's offsets and scan partials) lives in a fixed 16 MiB device array
+ * that each device holds for the life of the process, so a call allocates
+ * nothing.  At most ~590,000 seeds per call (PA_E_CAPACITY beyond, with the
+ * limit in the message).  Calls that overlap on one device share that array
+ * and race: run them on one stream.  Kernel timing (pa_gpu_timing_enable)
+ * creates its events on demand; leave it off in such callers. */
+pa_status   pa_gpu_tracer_trace_union_round_noalloc(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
+                                                    size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
+                                                    pa_particle* d_continuation, size_t* n_continuation,
+                                                    pa_color_mode mode, float parameter, float* d_vertices,
+                                                    size_t vertex_capacity, size_t* n_vertices, int32_t* d_indices,
+                                                    size_t index_capacity, size_t* n_indices, float* d_colors,
+                                                    float* d_endpoints, int32_t* d_seed_steps,
+                                                    uint64_t* total_steps);
 /* Counted prune of step-major slots (row stride n seeds) into the pruned
  * particle-major layout; same outputs as pa_gpu_prune_counted. */
 pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots, size_t n,

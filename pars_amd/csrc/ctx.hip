@@ -117,6 +117,13 @@ pa_status pa_gpu_ctx_create(int device, pa_gpu_ctx** out) {
     delete ctx;
     return PA_E_NOMEM;
   }
+  if (static_scratch_setup(ctx) != PA_OK) {
+    (void)hipHostFree(ctx->pinned);
+    (void)hipFree(ctx->counters);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PA_E_DEVICE;
+  }
   *out = ctx;
   return PA_OK;
 }

@@ -88,6 +88,7 @@ struct pa_gpu_ctx {
   size_t scratch_bytes = 0;
   void* counters = nullptr;  // 4 KiB of device counters
   void* pinned = nullptr;    // 4 KiB of pinned host memory for counter readback
+  void* static_scratch = nullptr;  // the device's fixed no-allocation workspace (never freed)
 };
 
 struct pa_gpu_tracer {
@@ -166,6 +167,19 @@ pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_
                         int32_t* d_indices, size_t index_capacity, float* d_colors, void* ws, int64_t* totals,
                         unsigned* flags);
 size_t finish_ws_bytes(int64_t n);
+// The fused round (trace on the hybrid engine + prune + index + colour)
+// with the caller's device workspace of union_round_ws_bytes(n) bytes; the
+// rest is pa_gpu_tracer_trace_union_round.
+size_t union_round_ws_bytes(int64_t n);
+pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n, size_t region_slots,
+                      float* d_slots, int32_t* d_vertex_counts, pa_particle* d_continuation, size_t* n_continuation,
+                      pa_color_mode mode, float parameter, float* d_vertices, size_t vertex_capacity,
+                      size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
+                      float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws);
+// Fixed workspace of the no-allocation fused round (noalloc.hip): its
+// device address on the context's device, looked up once per device at
+// context creation.
+pa_status static_scratch_setup(pa_gpu_ctx* ctx);
 // The colour stage's kernel (one lane per index).
 pa_status color_launch(pa_gpu_ctx* ctx, const float4* d_vertices, const int32_t* d_indices, int64_t n_indices,
                        int mode, float param, float4* d_colors);

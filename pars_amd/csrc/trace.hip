@@ -882,6 +882,27 @@ pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d
                                           size_t* n_indices, float* d_colors, float* d_endpoints,
                                           int32_t* d_seed_steps, uint64_t* total_steps) {
   if (!t || !n_vertices || !n_indices) return PA_E_INVALID;
+  void* ws = nullptr;
+  if (n > 0 && n <= size_t(INT32_MAX)) PA_TRY(scratch(t->ctx, union_round_ws_bytes(int64_t(n)), &ws));
+  return union_round(t, d_particles, n, region_slots, d_slots, d_vertex_counts, d_continuation, n_continuation, mode,
+                     parameter, d_vertices, vertex_capacity, n_vertices, d_indices, index_capacity, n_indices,
+                     d_colors, d_endpoints, d_seed_steps, total_steps, ws);
+}
+
+}  // extern "C"
+
+namespace pa_amd {
+
+size_t union_round_ws_bytes(int64_t n) {
+  return (size_t(n) * sizeof(int32_t) + 255) / 256 * 256 + finish_ws_bytes(n);
+}
+
+pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n, size_t region_slots,
+                      float* d_slots, int32_t* d_vertex_counts, pa_particle* d_continuation, size_t* n_continuation,
+                      pa_color_mode mode, float parameter, float* d_vertices, size_t vertex_capacity,
+                      size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
+                      float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws) {
+  if (!t || !n_vertices || !n_indices) return PA_E_INVALID;
   pa_gpu_ctx* ctx = t->ctx;
   *n_vertices = 0;
   *n_indices = 0;
@@ -895,9 +916,8 @@ pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d
       mode != PA_COLOR_HSV_CONSTANT_V && mode != PA_COLOR_RGB && mode != PA_COLOR_MAGNITUDE)
     return fail(ctx, PA_E_INVALID, "unknown color mode");
   if (n > size_t(INT32_MAX)) return fail(ctx, PA_E_INVALID, "round: at most 2^31-1 seeds per call");
-  void* ws = nullptr;
+  if (!ws) return fail(ctx, PA_E_INVALID, "round: no workspace");
   const size_t pair_bytes = (n * sizeof(int32_t) + 255) / 256 * 256;
-  PA_TRY(scratch(ctx, pair_bytes + finish_ws_bytes(int64_t(n)), &ws));
   auto* pairs = static_cast<int32_t*>(ws);
   // trace and finish are queued back to back; one read-back at the end
   // carries the step count, the continuation count, the validation flags
@@ -925,9 +945,6 @@ pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d
   return PA_OK;
 }
 
-}  // extern "C"
-
-namespace pa_amd {
 namespace {
 pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_particles, size_t n,
                            size_t region_slots, pa_slot_order order, float* d_vertices, int32_t* d_vertex_counts,
