@@ -533,6 +533,14 @@ def main():
                     algorithmic_bytes_per_step=GATHER_BYTES[st0] + STORE_BYTES,
                     steps_per_launch=steps_per_pass, avg_launch_ms=base["trace_ms"],
                     effective_gather_gbs=steps_per_pass * GATHER_BYTES[st0] / (base["trace_ms"] * 1e-3) / 1e9)
+    if traffic and traffic.get("avg_launch_ns"):
+        # the committed rocprofv3 summary of the same kernel and workload
+        prof_ms = traffic["avg_launch_ns"] * 1e-6
+        roofline["profile"] = dict(source=traffic.get("source"), avg_launch_ms=prof_ms,
+                                   frac=alg_bytes / (prof_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   duration_ratio=base["trace_ms"] / prof_ms,
+                                   l2_hit_pct=traffic.get("l2_hit_pct"), valu_busy_pct=traffic.get("valu_busy_pct"),
+                                   dram_gbs=traffic["bytes_per_launch"] / (prof_ms * 1e-3) / 1e9)
 
     extra = {}
     for form in forms:
