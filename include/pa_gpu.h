@@ -189,6 +189,24 @@ pa_status   pa_gpu_field_synthesize(pa_gpu_ctx* ctx, int kind, const double* par
 typedef enum pa_layout { PA_LAYOUT_NODES = 0, PA_LAYOUT_CELLS = 1 } pa_layout;
 pa_status   pa_gpu_field_set_layout(pa_gpu_ctx* ctx, pa_gpu_field* field, pa_layout layout);
 
+/* data_io::load_vector_field (pa/source/stages/data_io.cpp:91-115): read
+ * one rank's ghosted block from an HDF5 file (dataset "vectors"
+ * [X][Y][Z][3] float; file attribute "spacing" float[3]) into host memory.
+ * h_xyz receives ghosted[0] * ghosted[1] * ghosted[2] * 3 floats, C order,
+ * from the hyperslab at offset (the rank's integer offset); spacing_out is
+ * the attribute divided by its largest component, offset_out =
+ * float(offset) * spacing and size_out = float(block) * spacing, FP32 as
+ * data_io.cpp:108-114 computes them.  libhdf5 1.10+ is opened at run time
+ * (PA_E_UNSUPPORTED without it).  Needs no GPU.  error (optional)
+ * receives a message. */
+pa_status   pa_gpu_read_vector_block(const char* path, const int64_t offset[3], const int32_t ghosted[3],
+                                     const int32_t block[3], float* h_xyz, float offset_out[3], float size_out[3],
+                                     float spacing_out[3], char* error, size_t error_len);
+/* The same block straight into a device field of the given storage (read
+ * into pinned memory, uploaded and encoded on the device). */
+pa_status   pa_gpu_field_load(pa_gpu_ctx* ctx, const char* path, const int64_t offset[3], const int32_t ghosted[3],
+                              const int32_t block[3], pa_storage storage, pa_gpu_field** out);
+
 /* Decoded FP32 node values back to the host, [d0][d1][d2][3]. */
 pa_status   pa_gpu_field_read(pa_gpu_ctx* ctx, const pa_gpu_field* field, float* h_xyz);
 pa_status   pa_gpu_field_bytes(const pa_gpu_field* field, size_t* bytes);
@@ -448,6 +466,59 @@ pa_status   pa_gpu_index_generate(pa_gpu_ctx* ctx, const float* d_vertices, size
 pa_status   pa_gpu_color_generate(pa_gpu_ctx* ctx, const float* d_vertices, size_t n_vertices,
                                   const int32_t* d_indices, size_t n_indices, pa_color_mode mode, float parameter,
                                   float* d_colors);
+
+/* ------------------------------------------------------------- hand-overs */
+/* One chunk of a round's output, as pars hands it to the renderer:
+ * ray_tracer::set_integral_curves (pars/source/stages/ray_tracer.cpp:156-212)
+ * shares each integral_curves chunk with OSPRay as vertex (OSP_FLOAT3A:
+ * vec4 FP32, w = 0), vertex.color (OSP_FLOAT4) and index (OSP_INT) buffers.
+ * The pointers are device (HBM) pointers into the round's outputs. */
+typedef struct pa_curve_chunk {
+  float*   vertices;       /* n_vertices vec4 */
+  float*   colors;         /* n_vertices vec4 */
+  int32_t* indices;        /* n_indices, relative to vertices */
+  size_t   n_vertices;
+  size_t   n_indices;
+  size_t   first_particle; /* seeds [first_particle, first_particle + n_particles) */
+  size_t   n_particles;
+} pa_curve_chunk;
+/* Split a fused round's outputs (pa_gpu_tracer_trace_union_round) into the
+ * reference's chunks: particle_tracer::compute_round_info
+ * (pa/source/stages/particle_tracer.cpp:210-222) gives each chunk
+ * int32_max / sizeof(vector4) / R particles so that no chunk exceeds
+ * 134,217,727 vertices and int32 indices stay valid; here R is region_slots
+ * (a seed's segments never exceed it) and max_vertices_per_chunk (0 = the
+ * reference's 134,217,727) may lower the cap.  Indices are rewritten in
+ * place relative to their chunk, and each chunk's index list is what
+ * index_generator::generate (index_generator.cpp:10-24) gives on that chunk
+ * alone: the pair joining one chunk's last vertex to the next chunk's first
+ * belongs to neither.  Colours at the positions color_generator writes per
+ * chunk are unchanged.  Call once per round; *n_chunks is set even when
+ * max_chunks is too small (PA_E_CAPACITY). */
+pa_status   pa_gpu_round_chunks(pa_gpu_ctx* ctx, const int32_t* d_vertex_counts, size_t n, size_t region_slots,
+                                size_t max_vertices_per_chunk, float* d_vertices, size_t n_vertices,
+                                int32_t* d_indices, size_t n_indices, float* d_colors, pa_curve_chunk* chunks,
+                                size_t max_chunks, size_t* n_chunks);
+/* Particles as the records boost::mpi sends: the field sequence of
+ * particle::serialize (pa/include/pa/math/particle.hpp:16-28), position[0..3],
+ * remaining_iterations, vector_field_index, original_rank,
+ * original_voxel[0..2], 4 bytes each, 40 bytes per particle, packed.  The
+ * hand-off lists of pa_gpu_tracer_trace and the surplus lists below are
+ * turned into them on the device so the host's MPI sends straight from a
+ * device (or pinned) buffer. */
+pa_status   pa_gpu_particles_to_wire(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, void* d_wire);
+pa_status   pa_gpu_particles_from_wire(pa_gpu_ctx* ctx, const void* d_wire, size_t n, pa_particle* d_particles);
+/* The particle side of particle_tracer::load_balance_distribute
+ * (pa/source/stages/particle_tracer.cpp:168-184); the host keeps the count
+ * exchange and the averages (:63-166).  For neighbour slot k = 0..5 in
+ * order, the last surplus_counts[k] particles of the list as it stands
+ * (none if that is more than it holds) are written, in order, as serialize
+ * records to d_surplus_wire + surplus_offsets[k] records with
+ * vector_field_index = k ^ 1 (the receiver's slot for this process); the
+ * list keeps its first *n_remaining particles. */
+pa_status   pa_gpu_load_balance_take(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n,
+                                     const size_t surplus_counts[6], void* d_surplus_wire,
+                                     size_t surplus_offsets[7], size_t* n_remaining);
 
 /* ----------------------------------------------------------- advector / FTLE */
 /* pa::particle_advector::advect, one round in place over one block

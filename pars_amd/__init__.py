@@ -251,6 +251,16 @@ class VectorField:
         return cls(ctx, h, a.shape[:3], offset, size, spacing, storage)
 
     @classmethod
+    def load(cls, ctx: Context, path: str, offset, ghosted, block, storage=FP32):
+        """data_io::load_vector_field (data_io.cpp:91-115): one rank's ghosted HDF5 hyperslab straight
+        into device bricks; offset is the rank's integer offset, ghosted and block its node counts."""
+        _, off, size, sp = read_vector_block(path, offset, ghosted, block, want_data=False)
+        h = C.c_void_p()
+        ctx.check(lib().pa_gpu_field_load(ctx.handle, path.encode(), (C.c_int64 * 3)(*[int(x) for x in offset]),
+                                          _i3(ghosted), _i3(block), C.c_int(storage), C.byref(h)), "field_load")
+        return cls(ctx, h, ghosted, off, size, sp, storage)
+
+    @classmethod
     def synthesize(cls, ctx: Context, kind: str, params, dims, origin, h, offset, size, spacing, storage=FP32):
         """ABC flow (params A, B, C) or Fourier modes (rows k0 k1 k2 c0 c1 c2 phi) on the device."""
         p = np.ascontiguousarray(params, np.float64).ravel()
@@ -711,3 +721,77 @@ class ftle_map_generator:
                                                   _f3(self.spacing), C.c_double(self.time), C.c_int(self.mode),
                                                   C.c_void_p(out.ptr), C.c_void_p(lm.ptr) if lm else None), "ftle")
         return out, lm
+
+
+# ----------------------------------------------------------------- hand-overs
+# particle::serialize field order (pa/include/pa/math/particle.hpp:16-28): the
+# 40-byte records boost::mpi sends.
+WIRE_DTYPE = np.dtype([("position", "<f4", (4,)), ("remaining_iterations", "<i4"), ("vector_field_index", "<i4"),
+                       ("original_rank", "<i4"), ("original_voxel", "<i4", (3,))])
+assert WIRE_DTYPE.itemsize == 40
+
+
+class _CurveChunk(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("colors", C.c_void_p), ("indices", C.c_void_p), ("n_vertices", C.c_size_t),
+                ("n_indices", C.c_size_t), ("first_particle", C.c_size_t), ("n_particles", C.c_size_t)]
+
+
+def round_chunks(ctx: Context, counts: DeviceBuffer, n: int, region_slots: int, vertices: DeviceBuffer, n_vertices: int,
+                 indices: DeviceBuffer, n_indices: int, colors: DeviceBuffer = None, max_vertices_per_chunk: int = 0,
+                 max_chunks: int = 4096):
+    """A fused round's outputs as the renderer's chunks (pa_gpu_round_chunks; ray_tracer.cpp:156-212):
+    a list of dicts with device pointers and sizes; indices are rewritten relative to each chunk."""
+    arr = (_CurveChunk * max(1, max_chunks))()
+    nc = C.c_size_t()
+    ctx.check(lib().pa_gpu_round_chunks(
+        ctx.handle, C.c_void_p(counts.ptr), C.c_size_t(n), C.c_size_t(region_slots), C.c_size_t(max_vertices_per_chunk),
+        C.c_void_p(vertices.ptr), C.c_size_t(n_vertices), C.c_void_p(indices.ptr if indices is not None else None),
+        C.c_size_t(n_indices), C.c_void_p(colors.ptr if colors is not None else None), arr, C.c_size_t(max_chunks),
+        C.byref(nc)), "round_chunks")
+    return [dict(vertices=arr[c].vertices, colors=arr[c].colors, indices=arr[c].indices,
+                 n_vertices=arr[c].n_vertices, n_indices=arr[c].n_indices, first_particle=arr[c].first_particle,
+                 n_particles=arr[c].n_particles) for c in range(nc.value)]
+
+
+def particles_to_wire(ctx: Context, particles: "Particles") -> DeviceBuffer:
+    """Device particles as serialize records (WIRE_DTYPE), on the device."""
+    out = ctx.alloc(max(particles.n, 1) * 40)
+    ctx.check(lib().pa_gpu_particles_to_wire(ctx.handle, C.c_void_p(particles.ptr), C.c_size_t(particles.n),
+                                             C.c_void_p(out.ptr)), "particles_to_wire")
+    return out
+
+
+def particles_from_wire(ctx: Context, wire: DeviceBuffer, n: int) -> "Particles":
+    out = ctx.alloc(max(n, 1) * 48)
+    ctx.check(lib().pa_gpu_particles_from_wire(ctx.handle, C.c_void_p(wire.ptr), C.c_size_t(n), C.c_void_p(out.ptr)),
+              "particles_from_wire")
+    return Particles(out, n)
+
+
+def load_balance_take(ctx: Context, particles: "Particles", surplus_counts):
+    """load_balance_distribute's surplus lists (particle_tracer.cpp:168-184) as serialize records:
+    returns (wire buffer, offsets[7], particles kept)."""
+    counts = (C.c_size_t * 6)(*[int(c) for c in surplus_counts])
+    offs = (C.c_size_t * 7)()
+    keep = C.c_size_t()
+    total = sum(int(c) for c in surplus_counts)
+    wire = ctx.alloc(max(total, 1) * 40)
+    ctx.check(lib().pa_gpu_load_balance_take(ctx.handle, C.c_void_p(particles.ptr), C.c_size_t(particles.n), counts,
+                                             C.c_void_p(wire.ptr), offs, C.byref(keep)), "load_balance_take")
+    return wire, list(offs), keep.value
+
+
+def read_vector_block(path: str, offset, ghosted, block, want_data: bool = True):
+    """The host read of data_io::load_vector_field (data_io.cpp:91-115), no GPU needed:
+    returns (xyz [g0][g1][g2][3] float32, offset, size, spacing) as FP32 arrays."""
+    g = [int(x) for x in ghosted]
+    xyz = np.empty((g[0], g[1], g[2], 3), np.float32) if want_data else np.empty((2, 2, 2, 3), np.float32)
+    gg = g if want_data else [2, 2, 2]
+    off, size, sp = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()
+    err = C.create_string_buffer(512)
+    st = lib().pa_gpu_read_vector_block(path.encode(), (C.c_int64 * 3)(*[int(x) for x in offset]), _i3(gg),
+                                        _i3(block), xyz.ctypes.data_as(C.c_void_p), off, size, sp, err,
+                                        C.c_size_t(512))
+    if st != OK:
+        raise PaError(st, "read_vector_block: " + err.value.decode())
+    return xyz, np.array(off[:], np.float32), np.array(size[:], np.float32), np.array(sp[:], np.float32)

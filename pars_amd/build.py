@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PA_GPU_ARCH", "gfx950")
 
 SOURCES = ["ctx.hip", "field.hip", "trace.hip", "trace_coop.hip", "trace_quad.hip", "compact.hip", "color.hip",
-           "round.hip", "batch.hip", "ftle.hip", "noalloc.hip"]
+           "round.hip", "batch.hip", "ftle.hip", "noalloc.hip", "handover.hip", "io.hip"]
 
 FLAGS = [
     "--offload-arch=" + ARCH,
@@ -50,7 +50,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
