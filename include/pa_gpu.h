@@ -202,6 +202,9 @@ pa_status   pa_gpu_field_set_layout(pa_gpu_ctx* ctx, pa_gpu_field* field, pa_lay
 pa_status   pa_gpu_read_vector_block(const char* path, const int64_t offset[3], const int32_t ghosted[3],
                                      const int32_t block[3], float* h_xyz, float offset_out[3], float size_out[3],
                                      float spacing_out[3], char* error, size_t error_len);
+/* data_io::load_dimensions (pa/source/stages/data_io.cpp:36-40): the node
+ * counts of the dataset "vectors" [X][Y][Z][3].  Needs no GPU. */
+pa_status   pa_gpu_read_vector_dims(const char* path, int64_t dims[3], char* error, size_t error_len);
 /* The same block straight into a device field of the given storage (read
  * into pinned memory, uploaded and encoded on the device). */
 pa_status   pa_gpu_field_load(pa_gpu_ctx* ctx, const char* path, const int64_t offset[3], const int32_t ghosted[3],
@@ -499,6 +502,18 @@ pa_status   pa_gpu_round_chunks(pa_gpu_ctx* ctx, const int32_t* d_vertex_counts,
                                 size_t max_vertices_per_chunk, float* d_vertices, size_t n_vertices,
                                 int32_t* d_indices, size_t n_indices, float* d_colors, pa_curve_chunk* chunks,
                                 size_t max_chunks, size_t* n_chunks);
+/* Stand-in for the streamline ray tracer (OSPRay, out of scope): splats
+ * n vertices with their colours into a width x height RGBA8 image
+ * (d_rgba, 4 bytes per pixel, row 0 at the bottom, sRGB-encoded like
+ * OSPRay's OSP_FB_SRGBA framebuffer that ray_tracer::serialize sends,
+ * pars/source/stages/ray_tracer.cpp:240-253) through a pinhole camera at
+ * position looking along forward with up, fovy 68 degrees and aspect
+ * width / height (ray_tracer.cpp:63, 215-227); one pixel per vertex, the
+ * nearest wins; sentinel vertices are skipped; d_colors may be NULL
+ * (white).  Asynchronous. */
+pa_status   pa_gpu_render_points(pa_gpu_ctx* ctx, const float* d_vertices, const float* d_colors, size_t n,
+                                 const float position[3], const float forward[3], const float up[3], int32_t width,
+                                 int32_t height, uint8_t* d_rgba);
 /* Particles as the records boost::mpi sends: the field sequence of
  * particle::serialize (pa/include/pa/math/particle.hpp:16-28), position[0..3],
  * remaining_iterations, vector_field_index, original_rank,

@@ -795,3 +795,13 @@ def read_vector_block(path: str, offset, ghosted, block, want_data: bool = True)
     if st != OK:
         raise PaError(st, "read_vector_block: " + err.value.decode())
     return xyz, np.array(off[:], np.float32), np.array(size[:], np.float32), np.array(sp[:], np.float32)
+
+
+def read_vector_dims(path: str):
+    """data_io::load_dimensions (data_io.cpp:36-40): node counts of the HDF5 dataset "vectors"."""
+    dims = (C.c_int64 * 3)()
+    err = C.create_string_buffer(512)
+    st = lib().pa_gpu_read_vector_dims(path.encode(), dims, err, C.c_size_t(512))
+    if st != OK:
+        raise PaError(st, "read_vector_dims: " + err.value.decode())
+    return [int(d) for d in dims]

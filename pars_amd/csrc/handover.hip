@@ -9,6 +9,7 @@
 //    load_balance_distribute (particle_tracer.cpp:168-184) taken off the end
 //    of the device particle list.
 #include <algorithm>
+#include <cmath>
 #include <string>
 
 #include "pa_internal.hpp"
@@ -105,6 +106,60 @@ __global__ __launch_bounds__(kThreads) void k_take(const pa_particle* __restrict
 }
 
 unsigned blocks_of(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
+
+// Point-splat stand-in for the streamline ray tracer: a pinhole camera as
+// ray_tracer::set_camera / set_image_size configure OSPRay's (pos, dir, up,
+// fovy 68 degrees, aspect = width / height; ray_tracer.cpp:63, 215-227), one
+// pixel per vertex, nearest vertex wins (depth and colour packed into one
+// 64-bit key, atomicMin), colours encoded as sRGB 8-bit like OSPRay's
+// OSP_FB_SRGBA framebuffer; row 0 is the bottom row.
+struct Camera {
+  float pos[3], right[3], up[3], fwd[3];  // orthonormal basis
+  float tan_half;                         // tan(fovy / 2)
+  float aspect;
+  int w, h;
+};
+
+__global__ __launch_bounds__(kThreads) void k_fill_u64(unsigned long long* __restrict__ p, int64_t n,
+                                                       unsigned long long v) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ uint32_t srgb8(float c) {
+  c = fminf(fmaxf(c, 0.0f), 1.0f);
+  const float e = c <= 0.0031308f ? 12.92f * c : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+  return uint32_t(fminf(fmaxf(e * 255.0f + 0.5f, 0.0f), 255.0f));
+}
+
+__global__ __launch_bounds__(kThreads) void k_splat(const float4* __restrict__ v, const float4* __restrict__ col,
+                                                    int64_t n, const Camera cam,
+                                                    unsigned long long* __restrict__ zbuf) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = v[i];
+  // the sentinels of integral_curves.hpp:18-19 are not vertices
+  if (p.w == 0.0f && p.x == p.y && p.y == p.z && (p.x == -1.0f || p.x == -2.0f)) return;
+  const float d[3] = {p.x - cam.pos[0], p.y - cam.pos[1], p.z - cam.pos[2]};
+  const float z = d[0] * cam.fwd[0] + d[1] * cam.fwd[1] + d[2] * cam.fwd[2];
+  if (!(z > 1e-6f)) return;
+  const float x = (d[0] * cam.right[0] + d[1] * cam.right[1] + d[2] * cam.right[2]) / (z * cam.tan_half * cam.aspect);
+  const float y = (d[0] * cam.up[0] + d[1] * cam.up[1] + d[2] * cam.up[2]) / (z * cam.tan_half);
+  const float px = (x * 0.5f + 0.5f) * float(cam.w), py = (y * 0.5f + 0.5f) * float(cam.h);
+  if (!(px >= 0.0f && px < float(cam.w) && py >= 0.0f && py < float(cam.h))) return;
+  const float4 c = col ? col[i] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  const uint32_t rgba = srgb8(c.x) | srgb8(c.y) << 8 | srgb8(c.z) << 16 | uint32_t(fminf(fmaxf(c.w, 0.0f), 1.0f) * 255.0f + 0.5f) << 24;
+  const unsigned long long key = (static_cast<unsigned long long>(__float_as_uint(z)) << 32) | rgba;
+  atomicMin(&zbuf[int64_t(int(py)) * cam.w + int(px)], key);
+}
+
+__global__ __launch_bounds__(kThreads) void k_resolve(const unsigned long long* __restrict__ zbuf, int64_t n,
+                                                      uint32_t* __restrict__ rgba) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long k = zbuf[i];
+  rgba[i] = k == ~0ull ? 0u : uint32_t(k & 0xFFFFFFFFull);
+}
 
 }  // namespace
 }  // namespace pa_amd
@@ -218,6 +273,48 @@ pa_status pa_gpu_load_balance_take(pa_gpu_ctx* ctx, const pa_particle* d_particl
       hipLaunchKernelGGL(k_take, dim3(blocks_of(int64_t(take[k]))), dim3(kThreads), 0, ctx->stream, d_particles,
                          int64_t(from[k]), int64_t(take[k]), int32_t(k ^ 1),
                          static_cast<int32_t*>(d_surplus_wire) + 10 * surplus_offsets[k]);
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+
+pa_status pa_gpu_render_points(pa_gpu_ctx* ctx, const float* d_vertices, const float* d_colors, size_t n,
+                               const float position[3], const float forward[3], const float up[3], int32_t width,
+                               int32_t height, uint8_t* d_rgba) {
+  if (!ctx || !position || !forward || !up || !d_rgba || (n && !d_vertices)) return PA_E_INVALID;
+  if (width < 1 || height < 1 || int64_t(width) * height > (int64_t(1) << 28))
+    return fail(ctx, PA_E_INVALID, "render_points: image of 1 to 2^28 pixels");
+  Camera cam;
+  // basis: forward, then right = forward x up, then the true up = right x forward
+  float f[3] = {forward[0], forward[1], forward[2]};
+  double fl = std::sqrt(double(f[0]) * f[0] + double(f[1]) * f[1] + double(f[2]) * f[2]);
+  float r[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+  double rl = std::sqrt(double(r[0]) * r[0] + double(r[1]) * r[1] + double(r[2]) * r[2]);
+  if (!(fl > 0) || !(rl > 0)) return fail(ctx, PA_E_INVALID, "render_points: forward and up must not be parallel");
+  for (int i = 0; i < 3; ++i) {
+    cam.pos[i] = position[i];
+    cam.fwd[i] = float(f[i] / fl);
+    cam.right[i] = float(r[i] / rl);
+  }
+  cam.up[0] = cam.right[1] * cam.fwd[2] - cam.right[2] * cam.fwd[1];
+  cam.up[1] = cam.right[2] * cam.fwd[0] - cam.right[0] * cam.fwd[2];
+  cam.up[2] = cam.right[0] * cam.fwd[1] - cam.right[1] * cam.fwd[0];
+  cam.tan_half = float(std::tan(68.0 * 3.14159265358979323846 / 360.0));
+  cam.aspect = float(width) / float(height);
+  cam.w = width;
+  cam.h = height;
+  const int64_t px = int64_t(width) * height;
+  void* ws = nullptr;
+  PA_TRY(scratch(ctx, size_t(px) * 8, &ws));
+  auto* zbuf = static_cast<unsigned long long*>(ws);
+  launch_begin(ctx, "render");
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks_of(px)), dim3(kThreads), 0, ctx->stream, zbuf, px, ~0ull);
+  if (n)
+    hipLaunchKernelGGL(k_splat, dim3(blocks_of(int64_t(n))), dim3(kThreads), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(d_vertices), reinterpret_cast<const float4*>(d_colors),
+                       int64_t(n), cam, zbuf);
+  hipLaunchKernelGGL(k_resolve, dim3(blocks_of(px)), dim3(kThreads), 0, ctx->stream, zbuf, px,
+                     reinterpret_cast<uint32_t*>(d_rgba));
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;

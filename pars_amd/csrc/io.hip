@@ -175,6 +175,37 @@ pa_status read_block(const char* path, const int64_t offset[3], const int32_t gh
   return PA_OK;
 }
 
+pa_status read_dims(const char* path, int64_t dims_out[3], std::string& err) {
+  if (!path || !dims_out) {
+    err = "read_vector_dims: null argument";
+    return PA_E_INVALID;
+  }
+  Hdf5& h = hdf5();
+  if (!h.so) {
+    err = h.error;
+    return PA_E_UNSUPPORTED;
+  }
+  static std::mutex reader;
+  std::lock_guard<std::mutex> lock(reader);
+  Closer c{h};
+  if ((c.file = h.fopen(path, kReadOnly, kDefault)) < 0) {
+    err = std::string("cannot open ") + path;
+    return PA_E_INVALID;
+  }
+  if ((c.dset = h.dopen(c.file, "vectors", kDefault)) < 0) {
+    err = "no dataset 'vectors'";
+    return PA_E_INVALID;
+  }
+  c.fspace = h.dget_space(c.dset);
+  hsize_t dims[4] = {0, 0, 0, 0};
+  if (c.fspace < 0 || h.sget_ndims(c.fspace) != 4 || h.sget_dims(c.fspace, dims, nullptr) != 4 || dims[3] != 3) {
+    err = "'vectors' is not a [X][Y][Z][3] dataset";
+    return PA_E_INVALID;
+  }
+  for (int i = 0; i < 3; ++i) dims_out[i] = int64_t(dims[i]);
+  return PA_OK;
+}
+
 }  // namespace
 }  // namespace pa_amd
 
@@ -187,6 +218,16 @@ pa_status pa_gpu_read_vector_block(const char* path, const int64_t offset[3], co
                                   float spacing_out[3], char* error, size_t error_len) {
   std::string err;
   const pa_status s = read_block(path, offset, ghosted, block, h_xyz, offset_out, size_out, spacing_out, err);
+  if (error && error_len) {
+    std::strncpy(error, err.c_str(), error_len - 1);
+    error[error_len - 1] = '\0';
+  }
+  return s;
+}
+
+pa_status pa_gpu_read_vector_dims(const char* path, int64_t dims[3], char* error, size_t error_len) {
+  std::string err;
+  const pa_status s = read_dims(path, dims, err);
   if (error && error_len) {
     std::strncpy(error, err.c_str(), error_len - 1);
     error[error_len - 1] = '\0';
