@@ -47,10 +47,12 @@ def log(*a):
 class Pipeline:
     """Device buffers and calls of one step, reused across steps."""
 
-    def __init__(self, pa, ctx, fields, share, storage, interp, n_max, order="step", engine=None, fused=False):
+    def __init__(self, pa, ctx, fields, share, storage, interp, n_max, order="step", engine=None, fused=False,
+                 graph=False):
         self.pa, self.ctx, self.lib = pa, ctx, pa.lib()
         self.order = order
         self.fused = fused
+        self.graph = None
         self.share = share
         t = pa.particle_tracer(ctx)
         t.set_local_vector_field(fields[0])
@@ -77,9 +79,44 @@ class Pipeline:
         self.colors = ctx.alloc(self.cap * 16)
         loc = share.local
         self.seed_args = (pa._f3(loc.offset), pa._f3(loc.size), pa._f3(loc.spacing * np.float32(2)))
+        if graph:
+            self._capture(n_max)
+
+    def _capture(self, n_max):
+        """One pass (seed lattice + asynchronous fused round) captured as a graph; each step
+        replays it and reads the 48-byte result record back."""
+        pa, lib, ctx = self.pa, self.lib, self.ctx
+        b = dict(self.ub)
+        b.update(out_vertices=self.pruned, out_indices=self.indices, out_colors=self.colors)
+        self.result = ctx.alloc(48)
+        parts = pa.Particles(self.seeds, n_max)
+        o, s, st = self.seed_args
+        n = C.c_size_t()
+        ctx.check(lib.pa_gpu_seed_generate(ctx.handle, o, s, st, C.c_int32(R), C.c_int32(21),
+                                           C.c_void_p(self.seeds.ptr), C.c_size_t(n_max), C.byref(n)), "seeds")
+        self.tracer.trace_union_round_async(parts, self.region, b, self.result, 2, 0.75)  # sizes the workspace
+        ctx.synchronize()
+        with pa.Graph.capture(ctx) as g:
+            ctx.check(lib.pa_gpu_seed_generate(ctx.handle, o, s, st, C.c_int32(R), C.c_int32(21),
+                                               C.c_void_p(self.seeds.ptr), C.c_size_t(n_max), C.byref(n)), "seeds")
+            self.tracer.trace_union_round_async(parts, self.region, b, self.result, 2, 0.75)
+        self.graph = g
+        self.n_graph = n.value
+        self.host_result = np.zeros(1, pa.ROUND_RESULT_DTYPE)
 
     def step(self):
         pa, lib, ctx = self.pa, self.lib, self.ctx
+        if self.graph is not None:
+            self.graph.launch()
+            ctx.check(lib.pa_gpu_memcpy_d2h(ctx.handle, self.host_result.ctypes.data_as(C.c_void_p),
+                                            C.c_void_p(self.result.ptr), C.c_size_t(48)), "result")
+            r = self.host_result
+            ctx.check(lib.pa_gpu_round_result_check(ctx.handle, r.ctypes.data_as(C.c_void_p), C.c_size_t(self.cap),
+                                                    C.c_size_t(self.cap)), "round")
+            if r["n_continuation"][0]:
+                raise RuntimeError("segment region overflow")
+            return dict(particles=self.n_graph, steps=int(r["total_steps"][0]), vertices=int(r["n_vertices"][0]),
+                        indices=int(r["n_indices"][0]))
         n = C.c_size_t()
         o, s, st = self.seed_args
         ctx.check(lib.pa_gpu_seed_generate(ctx.handle, o, s, st, C.c_int32(R), C.c_int32(21),
@@ -129,8 +166,9 @@ class Pipeline:
         return dict(particles=n.value, steps=r["total_steps"], vertices=m.value, indices=ni.value)
 
 
-def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order, engine=None, fused=False):
-    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order, engine, fused)
+def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order, engine=None, fused=False,
+             graph=False):
+    p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order, engine, fused, graph)
     for _ in range(warmup):
         info = p.step()
     ctx.synchronize()
@@ -151,7 +189,7 @@ def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, or
     ctx.timing(False)
     kt = {k: ctx.kernel_time(k) for k in ("seeds", "trace_union", "prune_scan", "prune_counted", "index", "color",
                                           "round_counts", "round_finish")}
-    trace_ms = kt["trace_union"][0] / max(kt["trace_union"][1], 1)
+    trace_ms = kt["trace_union"][0] / kt["trace_union"][1] if kt["trace_union"][1] else None  # None: graph replays
     return p, dict(wall_s=wall, ms_per_step=1e3 * wall / steps, steps_per_pass=total // steps,
                    pass_latency_ms=dict(p50=1e3 * float(np.percentile(lat, 50)),
                                         p99=1e3 * float(np.percentile(lat, 99)), max=1e3 * max(lat)),
@@ -391,21 +429,54 @@ def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01, forms=FTLE_FORMS):
 def bench_batched(pa, ctx, cache=None):
     """Batched uneven work: 8 random-Fourier curl fields (64^3 .. 256^3, BF16),
     64 seed sets (log-uniform 1e3..1e6 seeds, 100..2000 steps, dt 0.01) in one
-    persistent work-queue kernel, launch shape from the autotuner."""
+    persistent work-queue kernel, launch shape from the autotuner; the round
+    (pool initialisation + run) captured as a graph and replayed."""
     from pars_amd import batch as B
 
     wl = B.make_workload()
     tun = B.autotune(ctx, wl, cache=cache)
     fields = B.build_fields(ctx, wl, pa.BF16, tun.layout)
     steps, sec, pool = B.run_batch(ctx, wl, fields, tun.waves_per_cu)  # warm-up
+    ctx.timing_reset()
+    ctx.timing(True)
     steps, sec, pool = B.run_batch(ctx, wl, fields, tun.waves_per_cu, pool=pool)
+    ctx.synchronize()
+    ctx.timing(False)
+    kt = ctx.kernel_time("pool_run")
+    kernel_ms = kt[0] / max(kt[1], 1)
     done = pool.request_done.download(np.int32, len(wl.sets))
     complete = bool(all(done[k] == s["size"] for k, s in enumerate(wl.sets)))
+    want_pos = pool.position.download(np.float32, 4 * wl.n)
+    # the same round captured: init (clearing the request counters) + run, replayed
+    t = B.make_tracer(ctx)
+    dev = pa.Particles.from_host(ctx, wl.particles)
+    rej, res = ctx.alloc(8), ctx.alloc(16)
+    with pa.Graph.capture(ctx) as g:
+        pa.pool_init_async(ctx, pool, dev, len(fields), rej, zero_requests=True)
+        pa.pool_run_async(pool, t, fields, res, n=wl.n, waves_per_cu=tun.waves_per_cu)
+    g.launch()  # warm-up replay
+    ctx.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    ctx.synchronize()
+    gsec = (time.perf_counter() - t0) / reps
+    gsteps = int(res.download(np.uint64, 2)[0])
+    graph_same = bool(gsteps == steps and pool.position.download(np.float32, 4 * wl.n).tobytes() == want_pos.tobytes())
+    g.close()
     for f in fields:
         f.release()
+    gather = 48  # BF16: 8 corners x 3 components x 2 B
     return dict(workload="8 curl fields of 48 random modes (64^3..256^3 nodes, BF16), 64 seed sets "
                          "(%d seeds), 100..2000 RK4 steps per set, dt 0.01, one persistent launch" % wl.n,
                 particles=wl.n, steps=steps, seconds=sec, steps_per_s=steps / sec, all_sets_complete=complete,
+                kernel_ms=kernel_ms,
+                roofline=dict(bound="hbm", kernel="k_pool_run<BF16 %s, RK4>" %
+                              ("cells" if tun.layout == pa.LAYOUT_CELLS else "nodes"),
+                              algorithmic_bytes_per_step=gather, achieved_gbs=steps * gather / (kernel_ms * 1e-3) / 1e9,
+                              peak=HBM_PEAK_GBS, frac=steps * gather / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
+                graph=dict(seconds_per_round=gsec, steps_per_s=gsteps / gsec, same_results=graph_same),
                 tuning=dict(layout="cells" if tun.layout == pa.LAYOUT_CELLS else "nodes",
                             waves_per_cu=tun.waves_per_cu, sample_rates=tun.table))
 
@@ -507,9 +578,10 @@ def main():
         engine = pa.ENGINE_LANE if "lane" in tokens else pa.ENGINE_HYBRID if order == "group8" else None
         fused = engine == pa.ENGINE_HYBRID and "stages" not in tokens
         p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order, engine,
-                          fused)
-        log("%s: %.3f ms/step, %.3e steps/s, trace kernel %.3f ms, %s" %
-            (form, res["ms_per_step"], res["value"], res["trace_ms"], res["info"]))
+                          fused, graph="graph" in tokens)
+        log("%s: %.3f ms/step, %.3e steps/s, trace kernel %s ms, %s" %
+            (form, res["ms_per_step"], res["value"], "%.3f" % res["trace_ms"] if res["trace_ms"] else "-",
+             res["info"]))
         results[form] = res
         pipes[form] = (p, st, interp)
 
@@ -546,14 +618,16 @@ def main():
     for form in forms:
         r = results[form]
         st = storage_of[form]
-        extra[form] = dict(value=r["value"], ms_per_step=r["ms_per_step"], trace_kernel_ms=r["trace_ms"],
+        tm = r["trace_ms"]
+        extra[form] = dict(value=r["value"], ms_per_step=r["ms_per_step"], trace_kernel_ms=tm,
                            kernel=kernel_name(form), pass_latency_ms=r["pass_latency_ms"],
-                           trace_steps_per_s=r["steps_per_pass"] / (r["trace_ms"] * 1e-3),
-                           steps_per_pass=r["steps_per_pass"], stage_ms=r["stage_ms"],
+                           trace_steps_per_s=r["steps_per_pass"] / (tm * 1e-3) if tm else None,
+                           steps_per_pass=r["steps_per_pass"], stage_ms=r["stage_ms"] if tm else None,
                            vs_fp32=r["value"] / base["value"],
-                           trace_vs_fp32=(r["steps_per_pass"] / r["trace_ms"]) /
-                                         (base["steps_per_pass"] / base["trace_ms"]),
-                           effective_gather_gbs=r["steps_per_pass"] * GATHER_BYTES[st] / (r["trace_ms"] * 1e-3) / 1e9)
+                           trace_vs_fp32=(r["steps_per_pass"] / tm) / (base["steps_per_pass"] / base["trace_ms"])
+                           if tm else None,
+                           effective_gather_gbs=r["steps_per_pass"] * GATHER_BYTES[st] / (tm * 1e-3) / 1e9
+                           if tm else None)
 
     orc = None
     if not args.no_errors or not args.no_cpu_baseline:

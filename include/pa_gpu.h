@@ -44,6 +44,7 @@ typedef enum pa_status {
 typedef struct pa_gpu_ctx pa_gpu_ctx;
 typedef struct pa_gpu_field pa_gpu_field;
 typedef struct pa_gpu_tracer pa_gpu_tracer;
+typedef struct pa_gpu_graph pa_gpu_graph;
 
 /* pa::particle (pa/include/pa/math/particle.hpp:30-35).  48 bytes: the
  * Eigen::Vector4f member gives the reference struct 16-byte alignment, so
@@ -390,6 +391,31 @@ pa_status   pa_gpu_tracer_trace_union_round_noalloc(pa_gpu_tracer* tracer, const
                                                     size_t index_capacity, size_t* n_indices, float* d_colors,
                                                     float* d_endpoints, int32_t* d_seed_steps,
                                                     uint64_t* total_steps);
+/* The fused round's result words, written on the device by the
+ * asynchronous form below (48 bytes). */
+typedef struct pa_round_result {
+  uint64_t total_steps;
+  uint32_t n_continuation;
+  uint32_t flags;         /* particle checks and finish waits; see pa_gpu_round_result_check */
+  uint64_t reserved_[2];
+  int64_t  n_vertices;    /* nothing was written when above vertex_capacity */
+  int64_t  n_indices;     /* nothing was written when above index_capacity */
+} pa_round_result;
+/* Extension: pa_gpu_tracer_trace_union_round without its read-back, for
+ * callers that keep several rounds in flight or capture them in a graph
+ * (pa_gpu_graph_begin): the counts land in d_result on the device and the
+ * call returns once the work is enqueued.  pa_gpu_round_result_check turns
+ * the result words (copied to the host) into the status the synchronous call
+ * returns.  Under capture the context's workspace must already be large
+ * enough (run the round once uncaptured first): PA_E_STATE otherwise. */
+pa_status   pa_gpu_tracer_trace_union_round_async(pa_gpu_tracer* tracer, const pa_particle* d_particles, size_t n,
+                                                  size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
+                                                  pa_particle* d_continuation, pa_color_mode mode, float parameter,
+                                                  float* d_vertices, size_t vertex_capacity, int32_t* d_indices,
+                                                  size_t index_capacity, float* d_colors, float* d_endpoints,
+                                                  int32_t* d_seed_steps, pa_round_result* d_result);
+pa_status   pa_gpu_round_result_check(pa_gpu_ctx* ctx, const pa_round_result* h_result, size_t vertex_capacity,
+                                      size_t index_capacity);
 /* Counted prune of step-major slots (row stride n seeds) into the pruned
  * particle-major layout; same outputs as pa_gpu_prune_counted. */
 pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots, size_t n,
@@ -444,6 +470,29 @@ pa_status   pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, si
 pa_status   pa_gpu_pool_run(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
                             const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
                             uint64_t* total_steps);
+
+/* Asynchronous forms of pa_gpu_pool_init and pa_gpu_pool_run (capturable):
+ * the rejected-entry count (uint32) and the (steps, rejected) pair (2 x
+ * uint64) land on the device; with zero_request_counters the request
+ * counters are cleared first (a kernel, not a memset). */
+pa_status   pa_gpu_pool_init_async(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
+                                   const pa_pool* pool, int32_t zero_request_counters, uint32_t* d_rejected);
+pa_status   pa_gpu_pool_run_async(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
+                                  const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
+                                  uint64_t* d_result);
+
+/* Extension: graph-captured rounds.  The asynchronous calls made on ctx
+ * between pa_gpu_graph_begin and pa_gpu_graph_end (seeds, the asynchronous
+ * round and pool calls, hand-over kernels) are recorded once and replayed by
+ * pa_gpu_graph_launch on ctx's stream: one submission per round, the same
+ * buffers and arguments every replay.  A call that reads back during
+ * capture makes pa_gpu_graph_end fail with PA_E_STATE.  Kernel timing must
+ * be off while capturing.  The round loop it serves is pars' per-request
+ * pipeline (pars/source/pipeline.cpp:153-260). */
+pa_status   pa_gpu_graph_begin(pa_gpu_ctx* ctx);
+pa_status   pa_gpu_graph_end(pa_gpu_ctx* ctx, pa_gpu_graph** out);
+pa_status   pa_gpu_graph_launch(pa_gpu_ctx* ctx, const pa_gpu_graph* graph);
+pa_status   pa_gpu_graph_destroy(pa_gpu_graph* graph);
 
 /* prune (particle_tracer.cpp:402-409): order-preserving removal of
  * invalid_vertex from one chunk.  *n_out receives the kept count. */
@@ -563,6 +612,14 @@ pa_status   pa_gpu_flow_map_generate(pa_gpu_tracer* tracer, const int32_t block_
 pa_status   pa_gpu_ftle_generate(pa_gpu_ctx* ctx, const float* d_flow_map, const int32_t dims[3],
                                  const float spacing[3], double time, pa_ftle_mode mode, float* d_ftle,
                                  double* d_lambda_max);
+
+/* The FP64 symmetric 3x3 eigen solve of pa_gpu_ftle_generate on its own
+ * (the role of Eigen's SelfAdjointEigenSolver at ftle_map_generator.cpp:
+ * 40-48): d_sym holds n matrices as (a00, a01, a02, a11, a12, a22), d_eigenvalues
+ * receives 3 eigenvalues per matrix, ascending.  Smith's trigonometric
+ * closed form; absolute error a few 1e-9 of the largest |eigenvalue| near
+ * repeated eigenvalues, ~1e-15 elsewhere. */
+pa_status   pa_gpu_sym3_eigenvalues(pa_gpu_ctx* ctx, const double* d_sym, size_t n, double* d_eigenvalues);
 
 #ifdef __cplusplus
 }

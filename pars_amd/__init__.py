@@ -496,6 +496,30 @@ class particle_tracer:
                     vertices=b["out_vertices"], n_vertices=nv.value, indices=b["out_indices"], n_indices=ni.value,
                     colors=col, endpoints=b["endpoints"], seed_steps=b["seed_steps"], total_steps=int(steps.value))
 
+    def trace_union_round_async(self, particles: Particles, region_slots: int, buffers, result: DeviceBuffer,
+                                mode: int = 2, parameter: float = 0.75):
+        """The fused round without its read-back (pa_gpu_tracer_trace_union_round_async): the
+        ROUND_RESULT_DTYPE record lands in `result` on the device; buffers as trace_union_round's."""
+        n = particles.n
+        b = buffers
+        vcap = max(1, n * region_slots)
+        self.ctx.check(lib().pa_gpu_tracer_trace_union_round_async(
+            self._h, C.c_void_p(particles.ptr), C.c_size_t(n), C.c_size_t(region_slots),
+            C.c_void_p(b["vertices"].ptr), C.c_void_p(b["counts"].ptr), C.c_void_p(b["cont"].ptr), C.c_int(mode),
+            C.c_float(parameter), C.c_void_p(b["out_vertices"].ptr), C.c_size_t(vcap),
+            C.c_void_p(b["out_indices"].ptr), C.c_size_t(vcap), C.c_void_p(b["out_colors"].ptr),
+            C.c_void_p(b["endpoints"].ptr), C.c_void_p(b["seed_steps"].ptr), C.c_void_p(result.ptr)),
+            "trace_union_round_async")
+
+    def round_buffers(self, n: int, region_slots: int):
+        """Device buffers of one fused round (slots, counts, outputs)."""
+        b = self.union_buffers(n, region_slots, endpoints=True)
+        vcap = max(1, n * region_slots)
+        b["out_vertices"] = self.ctx.alloc(vcap * 16)
+        b["out_indices"] = self.ctx.alloc(vcap * 4)
+        b["out_colors"] = self.ctx.alloc(vcap * 16)
+        return b
+
     def union_buffers(self, n: int, region_slots: int, endpoints: bool = False):
         n1 = max(n, 1)
         n8 = (n1 + 7) // 8 * 8  # the grouped order rounds the seeds up to groups of 8
@@ -805,3 +829,62 @@ def read_vector_dims(path: str):
     if st != OK:
         raise PaError(st, "read_vector_dims: " + err.value.decode())
     return [int(d) for d in dims]
+
+
+ROUND_RESULT_DTYPE = np.dtype([("total_steps", "<u8"), ("n_continuation", "<u4"), ("flags", "<u4"),
+                               ("reserved_", "<u8", (2,)), ("n_vertices", "<i8"), ("n_indices", "<i8")])
+assert ROUND_RESULT_DTYPE.itemsize == 48
+
+
+class Graph:
+    """A captured round (pa_gpu_graph_*): the asynchronous calls made on ctx inside
+    `with Graph.capture(ctx) as g:` are replayed by g.launch()."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+
+    @classmethod
+    def capture(cls, ctx: Context):
+        return _Capture(cls(ctx))
+
+    def launch(self):
+        self.ctx.check(lib().pa_gpu_graph_launch(self.ctx.handle, self._h), "graph_launch")
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().pa_gpu_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+
+class _Capture:
+    def __init__(self, g: Graph):
+        self.g = g
+
+    def __enter__(self):
+        self.g.ctx.check(lib().pa_gpu_graph_begin(self.g.ctx.handle), "graph_begin")
+        return self.g
+
+    def __exit__(self, exc_type, exc, tb):
+        st = lib().pa_gpu_graph_end(self.g.ctx.handle, C.byref(self.g._h))
+        if exc_type is None:
+            self.g.ctx.check(st, "graph_end")
+        return False
+
+
+def pool_init_async(ctx: Context, pool: "ParticlePool", particles: "Particles", n_fields: int,
+                    rejected: DeviceBuffer, zero_requests: bool = True):
+    pc = pool._c()
+    ctx.check(lib().pa_gpu_pool_init_async(ctx.handle, C.c_void_p(particles.ptr), C.c_size_t(particles.n),
+                                           C.c_int32(n_fields), C.byref(pc), C.c_int32(int(zero_requests)),
+                                           C.c_void_p(rejected.ptr)), "pool_init_async")
+    pool.n = max(pool.n, particles.n)
+
+
+def pool_run_async(pool: "ParticlePool", tracer: "particle_tracer", fields, result: DeviceBuffer, n: int = None,
+                   max_iterations: int = 0, waves_per_cu: int = 0):
+    arr = (C.c_void_p * len(fields))(*[f.handle for f in fields])
+    pc = pool._c()
+    pool.ctx.check(lib().pa_gpu_pool_run_async(tracer._h, arr, C.c_int32(len(fields)), C.byref(pc),
+                                               C.c_size_t(pool.n if n is None else n), C.c_int32(max_iterations),
+                                               C.c_int32(waves_per_cu), C.c_void_p(result.ptr)), "pool_run_async")

@@ -150,10 +150,21 @@ using namespace pa_amd;
 
 extern "C" {
 
-pa_status pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
+}  // extern "C"
+
+namespace {
+__global__ void k_zero_i32(int32_t* __restrict__ p, int n) {
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_copy_u32(const unsigned* __restrict__ src, unsigned* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+// Enqueues the pool's initialisation; the count of rejected entries is left
+// in the context counter 2080.
+pa_status pool_init_launch(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
                            const pa_pool* pool) {
-  if (!ctx || !pool) return PA_E_INVALID;
-  if (n == 0) return PA_OK;
   if (!d_particles || !pool->position || !pool->iteration || !pool->remaining || !pool->steps || !pool->status ||
       !pool->field)
     return PA_E_INVALID;
@@ -170,7 +181,20 @@ pa_status pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size
                      int(pool->n_requests), invalid);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+pa_status pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
+                           const pa_pool* pool) {
+  if (!ctx || !pool) return PA_E_INVALID;
+  if (n == 0) return PA_OK;
+  PA_TRY(pool_init_launch(ctx, d_particles, n, n_fields, pool));
+  auto* invalid = reinterpret_cast<unsigned int*>(static_cast<char*>(ctx->counters) + 2080);
   unsigned bad = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&bad, invalid, sizeof bad, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (bad)
@@ -179,12 +203,18 @@ pa_status pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, size
   return PA_OK;
 }
 
-pa_status pa_gpu_pool_run(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32_t n_fields, const pa_pool* pool,
-                          size_t n, int32_t max_iterations, int32_t waves_per_cu, uint64_t* total_steps) {
-  if (!t || !pool) return PA_E_INVALID;
+}  // extern "C"
+
+namespace {
+__global__ void k_pool_result(const unsigned long long* __restrict__ c, unsigned long long* __restrict__ out) {
+  if (threadIdx.x < 2) out[threadIdx.x] = c[1 + threadIdx.x];
+}
+
+// Enqueues the persistent pool kernel; its step count and rejected entries
+// are left in the context counters 2056 and 2064.
+pa_status pool_launch(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32_t n_fields, const pa_pool* pool,
+                      size_t n, int32_t max_iterations, int32_t waves_per_cu) {
   pa_gpu_ctx* ctx = t->ctx;
-  if (total_steps) *total_steps = 0;
-  if (n == 0) return PA_OK;
   if (!fields || n_fields < 1 || n_fields > kMaxFields) return fail(ctx, PA_E_INVALID, "pool: 1 to 16 fields");
   if (max_iterations < 0 || waves_per_cu < 0) return fail(ctx, PA_E_INVALID, "pool: negative budget or waves");
   if (pool->request && pool->request_done && pool->n_requests < 1)
@@ -224,13 +254,62 @@ pa_status pa_gpu_pool_run(pa_gpu_tracer* t, const pa_gpu_field* const* fields, i
   }, cfg.f[0].layout);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+pa_status pa_gpu_pool_run(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32_t n_fields, const pa_pool* pool,
+                          size_t n, int32_t max_iterations, int32_t waves_per_cu, uint64_t* total_steps) {
+  if (!t || !pool) return PA_E_INVALID;
+  pa_gpu_ctx* ctx = t->ctx;
+  if (total_steps) *total_steps = 0;
+  if (n == 0) return PA_OK;
+  PA_TRY(pool_launch(t, fields, n_fields, pool, n, max_iterations, waves_per_cu));
+  auto* counters = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2048);
   unsigned long long s[2] = {0, 0};
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(s, counters + 1, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (total_steps) *total_steps = s[0];
   if (s[1])
     return fail(ctx, PA_E_INVALID,
                 "pool: " + std::to_string(s[1]) + " entries name a field index >= n_fields; marked invalid");
+  return PA_OK;
+}
+
+pa_status pa_gpu_pool_init_async(pa_gpu_ctx* ctx, const pa_particle* d_particles, size_t n, int32_t n_fields,
+                                 const pa_pool* pool, int32_t zero_request_counters, uint32_t* d_rejected) {
+  if (!ctx || !pool || !d_rejected) return PA_E_INVALID;
+  auto* invalid = reinterpret_cast<unsigned int*>(static_cast<char*>(ctx->counters) + 2080);
+  if (zero_request_counters && pool->request_done && pool->n_requests > 0)
+    hipLaunchKernelGGL(k_zero_i32, dim3(unsigned((pool->n_requests + 255) / 256)), dim3(256), 0, ctx->stream,
+                       pool->request_done, int(pool->n_requests));
+  if (n == 0)
+    hipLaunchKernelGGL(k_pool_zero, dim3(1), dim3(64), 0, ctx->stream,
+                       reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2080));
+  else
+    PA_TRY(pool_init_launch(ctx, d_particles, n, n_fields, pool));
+  hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(64), 0, ctx->stream, invalid, d_rejected);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+
+pa_status pa_gpu_pool_run_async(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32_t n_fields,
+                                const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
+                                uint64_t* d_result) {
+  if (!t || !pool || !d_result) return PA_E_INVALID;
+  pa_gpu_ctx* ctx = t->ctx;
+  auto* counters = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2048);
+  if (n == 0) {
+    hipLaunchKernelGGL(k_pool_zero, dim3(1), dim3(64), 0, ctx->stream, counters);
+  } else {
+    PA_TRY(pool_launch(t, fields, n_fields, pool, n, max_iterations, waves_per_cu));
+  }
+  hipLaunchKernelGGL(k_pool_result, dim3(1), dim3(64), 0, ctx->stream, counters,
+                     reinterpret_cast<unsigned long long*>(d_result));
+  PA_HIP(ctx, hipGetLastError());
   return PA_OK;
 }
 

@@ -197,6 +197,7 @@ pa_status select(pa_gpu_ctx* ctx, Op op, int64_t n, size_t* n_out, const char* n
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   int64_t total = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&total, offs + t, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *n_out = size_t(total);
@@ -368,6 +369,7 @@ pa_status group_handoffs(pa_gpu_ctx* ctx, const int32_t* d_code, const pa_partic
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   int64_t h[kBuckets];
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(h, totals, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int b = 0; b < kBuckets; ++b) handoff->counts[b / 6][b % 6] = size_t(h[b]);
@@ -422,6 +424,7 @@ pa_status pa_gpu_prune_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t str
   PA_TRY(scan_rec<int32_t>(ctx, d_vertex_counts, int64_t(n), offs, sws));
   launch_end(ctx);
   int64_t total = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *n_out = size_t(total);
@@ -456,6 +459,7 @@ pa_status pa_gpu_prune_counted_chunked(pa_gpu_ctx* ctx, const float* d_slots, si
   PA_TRY(scan_rec<int32_t>(ctx, d_vertex_counts, int64_t(n), offs, sws));
   launch_end(ctx);
   int64_t total = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *n_out = size_t(total);
@@ -488,6 +492,7 @@ pa_status pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots,
   PA_TRY(scan_rec<int32_t>(ctx, d_vertex_counts, int64_t(n), offs, sws));
   launch_end(ctx);
   int64_t total = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *n_out = size_t(total);

@@ -15,7 +15,17 @@ pa_status check_hip(pa_gpu_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, e == hipErrorOutOfMemory ? PA_E_NOMEM : PA_E_DEVICE, m);
 }
 
+pa_status capture_guard(pa_gpu_ctx* ctx) {
+  if (!ctx->capturing) return PA_OK;
+  ctx->capture_failed = true;
+  return fail(ctx, PA_E_STATE, "this call reads back or copies host memory: not allowed while capturing a graph");
+}
+
 pa_status scratch(pa_gpu_ctx* ctx, size_t bytes, void** out) {
+  if (bytes > ctx->scratch_bytes && ctx->capturing) {
+    ctx->capture_failed = true;
+    return fail(ctx, PA_E_STATE, "the context's workspace would grow while capturing: run the call once first");
+  }
   if (bytes > ctx->scratch_bytes) {
     if (ctx->scratch) PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->scratch) PA_HIP(ctx, hipFree(ctx->scratch));
@@ -147,6 +157,7 @@ const char* pa_gpu_last_error(const pa_gpu_ctx* ctx) { return ctx ? ctx->error.c
 
 pa_status pa_gpu_synchronize(pa_gpu_ctx* ctx) {
   if (!ctx) return PA_E_INVALID;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return PA_OK;
 }
@@ -171,6 +182,7 @@ pa_status pa_gpu_malloc(pa_gpu_ctx* ctx, size_t bytes, void** d_out) {
   if (!ctx || !d_out) return PA_E_INVALID;
   *d_out = nullptr;
   if (bytes == 0) return PA_OK;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMalloc(d_out, bytes));
   return PA_OK;
 }
@@ -178,6 +190,7 @@ pa_status pa_gpu_malloc(pa_gpu_ctx* ctx, size_t bytes, void** d_out) {
 pa_status pa_gpu_free(pa_gpu_ctx* ctx, void* d_ptr) {
   if (!ctx) return PA_E_INVALID;
   if (!d_ptr) return PA_OK;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   PA_HIP(ctx, hipFree(d_ptr));
   return PA_OK;
@@ -186,6 +199,7 @@ pa_status pa_gpu_free(pa_gpu_ctx* ctx, void* d_ptr) {
 pa_status pa_gpu_memcpy_h2d(pa_gpu_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
   if (!ctx || (bytes && (!d_dst || !h_src))) return PA_E_INVALID;
   if (!bytes) return PA_OK;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return PA_OK;
@@ -194,6 +208,7 @@ pa_status pa_gpu_memcpy_h2d(pa_gpu_ctx* ctx, void* d_dst, const void* h_src, siz
 pa_status pa_gpu_memcpy_d2h(pa_gpu_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
   if (!ctx || (bytes && (!h_dst || !d_src))) return PA_E_INVALID;
   if (!bytes) return PA_OK;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return PA_OK;

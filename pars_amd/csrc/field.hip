@@ -214,6 +214,7 @@ pa_status pa_gpu_field_create_device(pa_gpu_ctx* ctx, const float* d_xyz, const 
 pa_status pa_gpu_field_create(pa_gpu_ctx* ctx, const float* h_xyz, const int32_t dims[3], const float offset[3],
                               const float size[3], const float spacing[3], pa_storage storage, pa_gpu_field** out) {
   if (!ctx || !h_xyz || !dims) return PA_E_INVALID;
+  PA_TRY(capture_guard(ctx));
   for (int i = 0; i < 3; ++i)
     if (dims[i] < 2) return fail(ctx, PA_E_INVALID, "a block needs at least 2 nodes per axis");
   const size_t n = size_t(dims[0]) * size_t(dims[1]) * size_t(dims[2]);
@@ -239,6 +240,7 @@ pa_status pa_gpu_field_synthesize(pa_gpu_ctx* ctx, int kind, const double* param
   PA_TRY(make_field(ctx, dims, offset, size, spacing, storage, out));
   double* d_params = nullptr;
   PA_HIP(ctx, hipMalloc(&d_params, sizeof(double) * size_t(n_params)));
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(d_params, params, sizeof(double) * size_t(n_params), hipMemcpyHostToDevice, ctx->stream));
   const int64_t n = int64_t(dims[0]) * dims[1] * dims[2];
   launch_begin(ctx, "field_synthesize");
@@ -253,6 +255,7 @@ pa_status pa_gpu_field_synthesize(pa_gpu_ctx* ctx, int kind, const double* param
 
 pa_status pa_gpu_field_read(pa_gpu_ctx* ctx, const pa_gpu_field* field, float* h_xyz) {
   if (!ctx || !field || !h_xyz) return PA_E_INVALID;
+  PA_TRY(capture_guard(ctx));
   const FieldDesc& d = field->desc;
   const int64_t n = int64_t(d.dims[0]) * d.dims[1] * d.dims[2];
   float* staging = nullptr;

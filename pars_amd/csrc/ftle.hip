@@ -97,10 +97,32 @@ __global__ __launch_bounds__(256) void k_ftle(const float* __restrict__ fm, int 
   if (lmax) lmax[v] = e[2];
 }
 
+__global__ __launch_bounds__(256) void k_eig_sym3(const double* __restrict__ a, int64_t n, double* __restrict__ e) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* m = a + 6 * i;
+  double v[3];
+  eig_sym3(m[0], m[1], m[2], m[3], m[4], m[5], v);
+  e[3 * i] = v[0];
+  e[3 * i + 1] = v[1];
+  e[3 * i + 2] = v[2];
+}
+
 }  // namespace
 }  // namespace pa_amd
 
 using namespace pa_amd;
+
+extern "C" pa_status pa_gpu_sym3_eigenvalues(pa_gpu_ctx* ctx, const double* d_sym, size_t n, double* d_eigenvalues) {
+  if (!ctx || (n && (!d_sym || !d_eigenvalues))) return PA_E_INVALID;
+  if (n == 0) return PA_OK;
+  launch_begin(ctx, "eig_sym3");
+  hipLaunchKernelGGL(k_eig_sym3, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, d_sym, int64_t(n),
+                     d_eigenvalues);
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
 
 extern "C" pa_status pa_gpu_ftle_generate(pa_gpu_ctx* ctx, const float* d_flow_map, const int32_t dims[3],
                                           const float spacing[3], double time, pa_ftle_mode mode, float* d_ftle,

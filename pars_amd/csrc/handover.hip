@@ -200,6 +200,7 @@ pa_status pa_gpu_round_chunks(pa_gpu_ctx* ctx, const int32_t* d_vertex_counts, s
   hipLaunchKernelGGL(k_chunk_bounds, dim3(unsigned((count + 63) / 64)), dim3(64), 0, ctx->stream, voff, int64_t(n),
                      ppic, count, d_indices, int64_t(n_indices), bounds);
   std::vector<int64_t> h(size_t(count) * 4);
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(h.data(), bounds, h.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (h[size_t(count) * 4 - 3] != int64_t(n_vertices)) {

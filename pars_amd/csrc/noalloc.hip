@@ -86,7 +86,7 @@ pa_status pa_gpu_tracer_trace_union_round_noalloc(pa_gpu_tracer* t, const pa_par
   }
   return union_round(t, d_particles, n, region_slots, d_slots, d_vertex_counts, d_continuation, n_continuation, mode,
                      parameter, d_vertices, vertex_capacity, n_vertices, d_indices, index_capacity, n_indices,
-                     d_colors, d_endpoints, d_seed_steps, total_steps, ctx->static_scratch);
+                     d_colors, d_endpoints, d_seed_steps, total_steps, ctx->static_scratch, nullptr);
 }
 
 }  // extern "C"

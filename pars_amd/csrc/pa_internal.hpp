@@ -14,6 +14,7 @@
 
 static_assert(sizeof(pa_particle) == 48, "pa_particle must match pa::particle (48 B)");
 static_assert(sizeof(pa_round_info) == 40, "pa_round_info layout");
+static_assert(sizeof(pa_round_result) == 48, "pa_round_result: the six result words of the round counters");
 
 namespace pa_amd {
 
@@ -89,6 +90,8 @@ struct pa_gpu_ctx {
   void* counters = nullptr;  // 4 KiB of device counters
   void* pinned = nullptr;    // 4 KiB of pinned host memory for counter readback
   void* static_scratch = nullptr;  // the device's fixed no-allocation workspace (never freed)
+  bool capturing = false;          // between pa_gpu_graph_begin and pa_gpu_graph_end
+  bool capture_failed = false;     // a call that reads back was made while capturing
 };
 
 struct pa_gpu_tracer {
@@ -111,8 +114,13 @@ namespace pa_amd {
 
 pa_status fail(pa_gpu_ctx* ctx, pa_status s, const std::string& msg);
 pa_status check_hip(pa_gpu_ctx* ctx, hipError_t e, const char* what);
-// Grow-only scratch on the context.
+// Grow-only scratch on the context (PA_E_STATE if it would grow while the
+// stream is being captured).
 pa_status scratch(pa_gpu_ctx* ctx, size_t bytes, void** out);
+// Calls that read back or copy from pageable host memory cannot be part of
+// a captured graph: they fail with PA_E_STATE while the context captures,
+// and the capture is then discarded by pa_gpu_graph_end.
+pa_status capture_guard(pa_gpu_ctx* ctx);
 // Timing brackets around a launch.
 void launch_begin(pa_gpu_ctx* ctx, const char* name);
 void launch_end(pa_gpu_ctx* ctx);
@@ -175,7 +183,8 @@ pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n
                       float* d_slots, int32_t* d_vertex_counts, pa_particle* d_continuation, size_t* n_continuation,
                       pa_color_mode mode, float parameter, float* d_vertices, size_t vertex_capacity,
                       size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
-                      float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws);
+                      float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws,
+                      pa_round_result* d_result);
 // Fixed workspace of the no-allocation fused round (noalloc.hip): its
 // device address on the context's device, looked up once per device at
 // context creation.

@@ -459,6 +459,7 @@ extern "C" pa_status pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d
   PA_TRY(scan_i32(ctx, d_vertex_counts, int64_t(n), offs, sws));
   launch_end(ctx);
   int64_t total = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *n_out = size_t(total);

@@ -522,6 +522,7 @@ pa_status scan_particles(pa_gpu_ctx* ctx, const TraceConfig& cfg, const pa_parti
                      c + 1, c + 2);
   PA_HIP(ctx, hipGetLastError());
   int h[3];
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(h, c, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (h[1] < -1 || h[2] > 5) return fail(ctx, PA_E_INVALID, "vector_field_index outside -1..5");
@@ -813,6 +814,7 @@ pa_status pa_gpu_tracer_trace(pa_gpu_tracer* t, const pa_particle* d_particles, 
   void* ws = nullptr;
   PA_TRY(scratch(ctx, table + tmp_bytes + group_bytes, &ws));
   char* w = static_cast<char*>(ws);
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(w, d_chunks, nchunks * sizeof(void*), hipMemcpyHostToDevice, ctx->stream));
   pa_particle* tmp = handoff ? reinterpret_cast<pa_particle*>(w + table) : nullptr;
   int32_t* code = handoff ? reinterpret_cast<int32_t*>(w + table + n * sizeof(pa_particle)) : nullptr;
@@ -849,6 +851,7 @@ pa_status pa_gpu_tracer_trace(pa_gpu_tracer* t, const pa_particle* d_particles, 
   PA_HIP(ctx, hipGetLastError());
   if (handoff) PA_TRY(group_handoffs(ctx, code, tmp, int64_t(n), handoff, w + table + tmp_bytes));
   unsigned long long s = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&s, steps, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (total_steps) *total_steps = s;
@@ -886,7 +889,39 @@ pa_status pa_gpu_tracer_trace_union_round(pa_gpu_tracer* t, const pa_particle* d
   if (n > 0 && n <= size_t(INT32_MAX)) PA_TRY(scratch(t->ctx, union_round_ws_bytes(int64_t(n)), &ws));
   return union_round(t, d_particles, n, region_slots, d_slots, d_vertex_counts, d_continuation, n_continuation, mode,
                      parameter, d_vertices, vertex_capacity, n_vertices, d_indices, index_capacity, n_indices,
-                     d_colors, d_endpoints, d_seed_steps, total_steps, ws);
+                     d_colors, d_endpoints, d_seed_steps, total_steps, ws, nullptr);
+}
+
+pa_status pa_gpu_tracer_trace_union_round_async(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n,
+                                                size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
+                                                pa_particle* d_continuation, pa_color_mode mode, float parameter,
+                                                float* d_vertices, size_t vertex_capacity, int32_t* d_indices,
+                                                size_t index_capacity, float* d_colors, float* d_endpoints,
+                                                int32_t* d_seed_steps, pa_round_result* d_result) {
+  if (!t || !d_result) return PA_E_INVALID;
+  pa_gpu_ctx* ctx = t->ctx;
+  void* ws = nullptr;
+  if (n > 0 && n <= size_t(INT32_MAX)) {
+    const size_t need = union_round_ws_bytes(int64_t(n));
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    PA_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone && need > ctx->scratch_bytes)
+      return fail(ctx, PA_E_STATE, "round under capture: run it once uncaptured first (its workspace must exist)");
+    PA_TRY(scratch(ctx, need, &ws));
+  }
+  return union_round(t, d_particles, n, region_slots, d_slots, d_vertex_counts, d_continuation, nullptr, mode,
+                     parameter, d_vertices, vertex_capacity, nullptr, d_indices, index_capacity, nullptr, d_colors,
+                     d_endpoints, d_seed_steps, nullptr, ws, d_result);
+}
+
+pa_status pa_gpu_round_result_check(pa_gpu_ctx* ctx, const pa_round_result* r, size_t vertex_capacity,
+                                    size_t index_capacity) {
+  if (!ctx || !r) return PA_E_INVALID;
+  PA_TRY(union_flags(ctx, r->flags));
+  if (size_t(r->n_vertices) > vertex_capacity) return fail(ctx, PA_E_CAPACITY, "round: vertex capacity too small");
+  if (size_t(r->n_indices) > index_capacity) return fail(ctx, PA_E_CAPACITY, "round: index capacity too small");
+  if (r->n_vertices > INT32_MAX) return fail(ctx, PA_E_CAPACITY, "round: indices are int32 (at most 2^31-1 vertices)");
+  return PA_OK;
 }
 
 }  // extern "C"
@@ -897,18 +932,34 @@ size_t union_round_ws_bytes(int64_t n) {
   return (size_t(n) * sizeof(int32_t) + 255) / 256 * 256 + finish_ws_bytes(n);
 }
 
+namespace {
+// The round's result words (counters 1024..1071) into the caller's buffer.
+__global__ void k_copy_result(const unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst) {
+  if (threadIdx.x < 6) dst[threadIdx.x] = src[threadIdx.x];
+}
+__global__ void k_zero_result(unsigned long long* __restrict__ dst) {
+  if (threadIdx.x < 6) dst[threadIdx.x] = 0ull;
+}
+}  // namespace
+
 pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n, size_t region_slots,
                       float* d_slots, int32_t* d_vertex_counts, pa_particle* d_continuation, size_t* n_continuation,
                       pa_color_mode mode, float parameter, float* d_vertices, size_t vertex_capacity,
                       size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
-                      float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws) {
-  if (!t || !n_vertices || !n_indices) return PA_E_INVALID;
+                      float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws,
+                      pa_round_result* d_result) {
+  if (!t || (!d_result && (!n_vertices || !n_indices))) return PA_E_INVALID;
   pa_gpu_ctx* ctx = t->ctx;
-  *n_vertices = 0;
-  *n_indices = 0;
+  if (n_vertices) *n_vertices = 0;
+  if (n_indices) *n_indices = 0;
   if (n == 0) {
     if (n_continuation) *n_continuation = 0;
     if (total_steps) *total_steps = 0;
+    if (d_result) {
+      hipLaunchKernelGGL(k_zero_result, dim3(1), dim3(64), 0, ctx->stream,
+                         reinterpret_cast<unsigned long long*>(d_result));
+      PA_HIP(ctx, hipGetLastError());
+    }
     return PA_OK;
   }
   if (!d_vertices || !d_indices || !d_colors) return PA_E_INVALID;
@@ -922,15 +973,24 @@ pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n
   // trace and finish are queued back to back; one read-back at the end
   // carries the step count, the continuation count, the validation flags
   // and the two totals
+  size_t ncont_ignored = 0;
   PA_TRY(trace_union_impl(t, PA_ENGINE_HYBRID, d_particles, n, region_slots, PA_SLOTS_GROUP8, d_slots,
-                          d_vertex_counts, d_continuation, n_continuation, d_endpoints, d_seed_steps, total_steps,
-                          pairs, true));
+                          d_vertex_counts, d_continuation, n_continuation ? n_continuation : &ncont_ignored,
+                          d_endpoints, d_seed_steps, total_steps, pairs, true));
   char* cbase = static_cast<char*>(ctx->counters);
   auto* totals = reinterpret_cast<int64_t*>(cbase + 1056);
   PA_TRY(finish_group8(ctx, d_slots, int64_t(n), int64_t(region_slots), d_vertex_counts, pairs, int(mode), parameter,
                        d_vertices, vertex_capacity, d_indices, index_capacity, d_colors,
                        static_cast<char*>(ws) + pair_bytes, totals, reinterpret_cast<unsigned*>(cbase + 1036)));
+  if (d_result) {  // no read-back: the caller reads the result words when it needs them
+    hipLaunchKernelGGL(k_copy_result, dim3(1), dim3(64), 0, ctx->stream,
+                       reinterpret_cast<const unsigned long long*>(cbase + 1024),
+                       reinterpret_cast<unsigned long long*>(d_result));
+    PA_HIP(ctx, hipGetLastError());
+    return PA_OK;
+  }
   unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(h, cbase + 1024, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   PA_TRY(union_flags(ctx, unsigned(h[1] >> 32)));
@@ -1029,6 +1089,7 @@ pa_status trace_union_impl(pa_gpu_tracer* t, int engine, const pa_particle* d_pa
   PA_HIP(ctx, hipGetLastError());
   if (deferred) return PA_OK;
   unsigned long long h[2] = {0, 0};
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(h, steps, 16, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (engine == PA_ENGINE_HYBRID) PA_TRY(union_flags(ctx, unsigned(h[1] >> 32)));
@@ -1066,6 +1127,7 @@ pa_status pa_gpu_advect(pa_gpu_tracer* t, pa_particle* d_particles, size_t n, in
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   unsigned long long s = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&s, steps, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (total_steps) *total_steps = s;
@@ -1125,6 +1187,7 @@ pa_status pa_gpu_flow_map_generate(pa_gpu_tracer* t, const int32_t block_size[3]
   }
   PA_HIP(ctx, hipGetLastError());
   unsigned long long s = 0;
+  PA_TRY(capture_guard(ctx));
   PA_HIP(ctx, hipMemcpyAsync(&s, steps, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (total_steps) *total_steps = s;

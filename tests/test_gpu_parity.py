@@ -665,3 +665,25 @@ def test_serving_alone_equals_mix(ctx, orc):
     q["vector_field_index"] = -1
     e, st = orc.endpoints(q, o, dt=S.DT, integrator=orc.RK4)
     assert reqs[0].endpoints.tobytes() == e.tobytes()
+
+
+def test_device_eigen_solver_on_lapack_fixtures(ctx):
+    """The FTLE's FP64 symmetric 3x3 eigen solve (Smith's closed form) on the
+    committed LAPACK (numpy.linalg.eigh) fixtures: near-degenerate,
+    rank-deficient and 1e-6..1e6-scaled matrices included.  Error relative to
+    the largest |eigenvalue| of each matrix."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "eigh3.npz"))
+    A, want = d["matrices"], d["eigenvalues"]
+    sym = np.ascontiguousarray(np.stack([A[:, 0, 0], A[:, 0, 1], A[:, 0, 2], A[:, 1, 1], A[:, 1, 2], A[:, 2, 2]], 1))
+    dev = ctx.upload(sym.astype(np.float64))
+    out = ctx.alloc(len(A) * 24)
+    ctx.check(pa.lib().pa_gpu_sym3_eigenvalues(ctx.handle, pa.C.c_void_p(dev.ptr), pa.C.c_size_t(len(A)),
+                                               pa.C.c_void_p(out.ptr)), "eig")
+    got = out.download(np.float64, 3 * len(A)).reshape(-1, 3)
+    scale = np.maximum(np.abs(want).max(1), 1e-300)
+    err = (np.abs(got - want).max(1) / scale)
+    print("eigen solver: max %.3g, median %.3g relative to the largest |eigenvalue|" % (err.max(), np.median(err)))
+    assert err.max() <= 1e-7
+    assert np.median(err) <= 1e-13
+    assert np.all(np.diff(got, axis=1) >= -1e-12 * scale[:, None])  # ascending
