@@ -45,6 +45,7 @@ typedef struct pa_gpu_ctx pa_gpu_ctx;
 typedef struct pa_gpu_field pa_gpu_field;
 typedef struct pa_gpu_tracer pa_gpu_tracer;
 typedef struct pa_gpu_graph pa_gpu_graph;
+typedef struct pa_gpu_server pa_gpu_server;
 
 /* pa::particle (pa/include/pa/math/particle.hpp:30-35).  48 bytes: the
  * Eigen::Vector4f member gives the reference struct 16-byte alignment, so
@@ -480,6 +481,34 @@ pa_status   pa_gpu_pool_init_async(pa_gpu_ctx* ctx, const pa_particle* d_particl
 pa_status   pa_gpu_pool_run_async(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
                                   const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
                                   uint64_t* d_result);
+
+/* Extension: device-resident serving loop.  No reference counterpart: pars
+ * serves one request at a time (pars/source/service.cpp:20-69).  A
+ * persistent kernel on the server's own stream takes the particles of every
+ * submitted request as soon as they are published and runs each to its end
+ * on its field (vector_field_index; the pool's one-block reference round),
+ * so requests of any size overlap on the device with no host round trip per
+ * pass; endpoints and step counts land in pinned host memory and a request's
+ * completion flag is raised by the lane that finishes its last particle.
+ * Ring space and request slots are admitted by capacity (PA_E_CAPACITY when
+ * full); a request's results equal its trace alone.  The kernel leaves
+ * after idle_polls empty polls (~30 us each; 0 = 64) and is launched again
+ * by submit or poll when work is waiting.  waves_per_cu 0 = 8. */
+pa_status   pa_gpu_serve_create(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
+                                size_t particle_capacity, int32_t request_slots, int32_t waves_per_cu,
+                                int32_t idle_polls, pa_gpu_server** out);
+/* Publish a request (host particles, copied): *request = its slot. */
+pa_status   pa_gpu_serve_submit(pa_gpu_server* server, const pa_particle* h_particles, size_t n, int32_t* request);
+/* Non-blocking: *done = 1 once every particle of the request has ended. */
+pa_status   pa_gpu_serve_poll(pa_gpu_server* server, int32_t request, int32_t* done);
+/* A done request's endpoints (n vec4) and step counts (either optional),
+ * then its slot and ring space are free again. */
+pa_status   pa_gpu_serve_collect(pa_gpu_server* server, int32_t request, float* h_endpoints, int32_t* h_steps,
+                                 uint64_t* steps_sum);
+/* Drain what is published, wait for the kernel to leave; total steps served
+ * and kernel launches so far. */
+pa_status   pa_gpu_serve_stop(pa_gpu_server* server, uint64_t* total_steps, uint64_t* launches);
+pa_status   pa_gpu_serve_destroy(pa_gpu_server* server);
 
 /* Extension: graph-captured rounds.  The asynchronous calls made on ctx
  * between pa_gpu_graph_begin and pa_gpu_graph_end (seeds, the asynchronous

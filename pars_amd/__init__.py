@@ -888,3 +888,53 @@ def pool_run_async(pool: "ParticlePool", tracer: "particle_tracer", fields, resu
     pool.ctx.check(lib().pa_gpu_pool_run_async(tracer._h, arr, C.c_int32(len(fields)), C.byref(pc),
                                                C.c_size_t(pool.n if n is None else n), C.c_int32(max_iterations),
                                                C.c_int32(waves_per_cu), C.c_void_p(result.ptr)), "pool_run_async")
+
+
+class DeviceServer:
+    """The device-resident serving loop (pa_gpu_serve_*): a persistent kernel takes each submitted
+    request's particles as soon as they are published; results land in pinned host memory."""
+
+    def __init__(self, ctx: Context, tracer: "particle_tracer", fields, particle_capacity: int,
+                 request_slots: int = 1024, waves_per_cu: int = 0, idle_polls: int = 0):
+        self.ctx = ctx
+        self._tracer = tracer
+        self._fields = list(fields)
+        arr = (C.c_void_p * len(fields))(*[f.handle for f in fields])
+        self._h = C.c_void_p()
+        ctx.check(lib().pa_gpu_serve_create(tracer._h, arr, C.c_int32(len(fields)), C.c_size_t(particle_capacity),
+                                            C.c_int32(request_slots), C.c_int32(waves_per_cu), C.c_int32(idle_polls),
+                                            C.byref(self._h)), "serve_create")
+
+    def submit(self, particles: np.ndarray):
+        """Returns the request slot, or None when the ring or the slots are full."""
+        p = np.ascontiguousarray(particles, PARTICLE_DTYPE)
+        slot = C.c_int32()
+        st = lib().pa_gpu_serve_submit(self._h, p.ctypes.data_as(C.c_void_p), C.c_size_t(len(p)), C.byref(slot))
+        if st == E_CAPACITY and "larger than the ring" not in lib().pa_gpu_last_error(self.ctx.handle).decode():
+            return None  # ring or slots full for now: try again once requests are collected
+        self.ctx.check(st, "serve_submit")
+        return slot.value
+
+    def done(self, slot: int) -> bool:
+        d = C.c_int32()
+        self.ctx.check(lib().pa_gpu_serve_poll(self._h, C.c_int32(slot), C.byref(d)), "serve_poll")
+        return bool(d.value)
+
+    def collect(self, slot: int, n: int, want_endpoints: bool = True):
+        ends = np.empty((n, 4), np.float32) if want_endpoints else None
+        steps = np.empty(n, np.int32)
+        total = C.c_uint64()
+        self.ctx.check(lib().pa_gpu_serve_collect(self._h, C.c_int32(slot),
+                                                  ends.ctypes.data_as(C.c_void_p) if ends is not None else None,
+                                                  steps.ctypes.data_as(C.c_void_p), C.byref(total)), "serve_collect")
+        return ends, steps, int(total.value)
+
+    def stop(self):
+        steps, launches = C.c_uint64(), C.c_uint64()
+        self.ctx.check(lib().pa_gpu_serve_stop(self._h, C.byref(steps), C.byref(launches)), "serve_stop")
+        return int(steps.value), int(launches.value)
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().pa_gpu_serve_destroy(self._h)
+            self._h = C.c_void_p()

@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PA_GPU_ARCH", "gfx950")
 
 SOURCES = ["ctx.hip", "field.hip", "trace.hip", "trace_coop.hip", "trace_quad.hip", "compact.hip", "color.hip",
-           "round.hip", "batch.hip", "ftle.hip", "noalloc.hip", "handover.hip", "io.hip", "graph.hip"]
+           "round.hip", "batch.hip", "ftle.hip", "noalloc.hip", "handover.hip", "io.hip", "graph.hip", "serve.hip"]
 
 FLAGS = [
     "--offload-arch=" + ARCH,

@@ -195,6 +195,59 @@ class Server:
         return steps_total, clock() - t0
 
 
+class ResidentServer:
+    """Serving on the device-resident loop (pa_gpu_serve_*): a request is published as soon as it
+    arrives and fits the ring; its latency runs from arrival to the host holding its endpoints."""
+
+    def __init__(self, ctx: Context, fields: Sequence[VectorField], capacity: Optional[int] = None,
+                 request_slots: int = 4096, waves_per_cu: int = 0, memory_fraction: float = 0.02):
+        from . import DeviceServer
+        self.ctx, self.fields = ctx, list(fields)
+        if capacity is None:  # admission by memory: pinned ring of 68 B per particle (in + out)
+            free = ctx.device_info()["free_bytes"]
+            capacity = min(int(free * memory_fraction) // 68, 1 << 24)
+        self.capacity = capacity
+        self.tracer = make_tracer(ctx)
+        self.server = DeviceServer(ctx, self.tracer, self.fields, capacity, request_slots, waves_per_cu)
+
+    def serve(self, requests: Sequence[Request], keep_endpoints: bool = True, clock=time.perf_counter,
+              timeout: float = 60.0):
+        waiting = list(requests)
+        active = {}  # slot -> request
+        steps_total = 0
+        t0 = clock()
+        while waiting or active:
+            now = clock() - t0
+            if now > timeout + (requests[-1].arrival if requests else 0.0):
+                raise RuntimeError("resident serving: %d requests still open after %.0f s" % (len(active), now))
+            while waiting and waiting[0].arrival <= now:
+                req = waiting[0]
+                if len(req.particles) > self.capacity:
+                    raise ValueError("request %d has more seeds than the ring holds" % req.rid)
+                slot = self.server.submit(req.particles)
+                if slot is None:
+                    break
+                waiting.pop(0)
+                req.admitted = clock() - t0
+                active[slot] = req
+            for slot in [s for s in active if self.server.done(s)]:
+                req = active.pop(slot)
+                ends, steps, total = self.server.collect(slot, len(req.particles), keep_endpoints)
+                req.endpoints = ends
+                req.steps = total
+                req.done = clock() - t0
+                steps_total += total
+            if not active and waiting:
+                wait = waiting[0].arrival - (clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.0005))
+        return steps_total, clock() - t0
+
+    def close(self):
+        self.server.stop()
+        self.server.close()
+
+
 def serve_one_at_a_time(ctx: Context, fields: Sequence[VectorField], requests: Sequence[Request],
                         keep_endpoints: bool = True, clock=time.perf_counter):
     """Baseline: each request alone, in arrival order, one launch to

@@ -687,3 +687,32 @@ def test_device_eigen_solver_on_lapack_fixtures(ctx):
     assert err.max() <= 1e-7
     assert np.median(err) <= 1e-13
     assert np.all(np.diff(got, axis=1) >= -1e-12 * scale[:, None])  # ascending
+
+
+@pytest.mark.parametrize("capacity", [60000, 4500])
+def test_resident_serving_alone_equals_mix(ctx, orc, capacity):
+    """The device-resident serving loop: requests published while others run
+    overlap on the device, a small ring admits them as space frees up, and
+    every request's endpoints and steps equal its result alone."""
+    from pars_amd import serve as S
+    grid = 64
+    field = S.make_field(ctx, pa.BF16, grid=grid, n_modes=8)
+    reqs = S.make_requests(14, rate=20000.0, grid=grid, steps=120, size_range=(64, 4000))
+    server = S.ResidentServer(ctx, [field], capacity=capacity, request_slots=8, waves_per_cu=4)
+    steps, wall = server.serve(reqs)
+    total, launches = server.server.stop()
+    server.close()
+    assert all(np.isfinite(r.done) for r in reqs)
+    assert steps == sum(r.steps for r in reqs) == total and launches >= 1
+    overlapped = sum(1 for a in reqs for b in reqs if a is not b and a.admitted < b.admitted < a.done)
+    assert overlapped > 0
+    alone = [S.Request(r.rid, 0.0, r.particles) for r in reqs]
+    S.serve_one_at_a_time(ctx, [field], alone)
+    for r, a in zip(reqs, alone):
+        assert r.endpoints.tobytes() == a.endpoints.tobytes(), r.rid
+        assert r.steps == a.steps
+    o = orc.Field(field.read(), field.offset, field.size, field.spacing)
+    q = reqs[3].particles.copy()
+    q["vector_field_index"] = -1
+    e, st = orc.endpoints(q, o, dt=S.DT, integrator=orc.RK4)
+    assert reqs[3].endpoints.tobytes() == e.tobytes()
