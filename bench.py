@@ -491,22 +491,34 @@ def bench_serving(pa, ctx, rates=(200.0, 400.0, 800.0, 1600.0), n_requests=60):
     f16 = S.make_field(ctx, pa.BF16).set_layout(pa.LAYOUT_CELLS)
     f32 = S.make_field(ctx, pa.FP32)
     server = S.Server(ctx, [f16], pass_iterations=200)
+    resident = S.ResidentServer(ctx, [f16])
     out = dict(workload="1024^3 BF16 curl field of 64 random modes (8.6 GB nodes, served from 64-B cell records) "
                         "resident; Poisson arrivals; 256..262144 seeds per request in 1-10%% boxes; 500 RK4 steps, "
-                        "dt 0.002; %d requests per rate; baseline: one request at a time on the FP32 node field"
-                        % n_requests, rates={})
+                        "dt 0.002; %d requests per rate; resident = device-resident loop, continuous = host-driven "
+                        "passes; baseline: one request at a time on the FP32 node field" % n_requests, rates={})
     warm = S.make_requests(4, 1000.0, seed=1)
     server.serve(warm, keep_endpoints=False)
+    resident.serve(S.make_requests(4, 1000.0, seed=1), keep_endpoints=False)
     for rate in rates:
+        res_reqs = S.make_requests(n_requests, rate, seed=int(rate))
+        rsteps, rwall = resident.serve(res_reqs, keep_endpoints=False)
         reqs = S.make_requests(n_requests, rate, seed=int(rate))
         steps, wall = server.serve(reqs, keep_endpoints=False)
         base = S.make_requests(n_requests, rate, seed=int(rate))
         bsteps, bwall = S.serve_one_at_a_time(ctx, [f32], base, keep_endpoints=False)
         out["rates"]["%g/s" % rate] = dict(
+            resident=dict(steps_per_s=rsteps / rwall, requests_per_s=n_requests / rwall,
+                          **S.latency_stats(res_reqs)),
             continuous=dict(steps_per_s=steps / wall, requests_per_s=n_requests / wall, **S.latency_stats(reqs)),
             one_at_a_time_fp32=dict(steps_per_s=bsteps / bwall, requests_per_s=n_requests / bwall,
                                     **S.latency_stats(base)))
         log("serving %g/s: %s" % (rate, out["rates"]["%g/s" % rate]))
+    # saturated: every request at once
+    sat = S.make_requests(n_requests, 1e9, seed=7)
+    ssteps, swall = resident.serve(sat, keep_endpoints=False)
+    out["saturated_resident"] = dict(steps_per_s=ssteps / swall, requests_per_s=n_requests / swall,
+                                     **S.latency_stats(sat))
+    resident.close()
     f16.release()
     f32.release()
     return out
@@ -529,6 +541,7 @@ def main():
                                        "fp32-cells-stages,fp32-cells-lane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-workloads", action="store_true", help="skip the FTLE, batched and serving workloads")
+    ap.add_argument("--workloads", default="ftle,batched,serving", help="which of the other workloads to run")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
     # one GPU's share of the host is 16 CPUs on the boxes this runs on
@@ -660,8 +673,11 @@ def main():
 
     workloads = {}
     if not args.no_workloads:
+        chosen = set(w.strip() for w in args.workloads.split(","))
         for name, fn in (("ftle", lambda: bench_ftle(pa, ctx, orc)), ("batched", lambda: bench_batched(pa, ctx)),
                          ("serving", lambda: bench_serving(pa, ctx))):
+            if name not in chosen:
+                continue
             t0 = time.perf_counter()
             try:
                 workloads[name] = fn()

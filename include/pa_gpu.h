@@ -488,12 +488,20 @@ pa_status   pa_gpu_pool_run_async(pa_gpu_tracer* tracer, const pa_gpu_field* con
  * submitted request as soon as they are published and runs each to its end
  * on its field (vector_field_index; the pool's one-block reference round),
  * so requests of any size overlap on the device with no host round trip per
- * pass; endpoints and step counts land in pinned host memory and a request's
- * completion flag is raised by the lane that finishes its last particle.
- * Ring space and request slots are admitted by capacity (PA_E_CAPACITY when
- * full); a request's results equal its trace alone.  The kernel leaves
- * after idle_polls empty polls (~30 us each; 0 = 64) and is launched again
- * by submit or poll when work is waiting.  waves_per_cu 0 = 8. */
+ * pass.  One wave of the grid reads the host's control words and mirrors
+ * them into device memory; workers claim ring entries from the device
+ * cursor, read the records from the pinned ring and leave endpoints and
+ * step counts in device memory; the request's last particle queues it, and
+ * exporter waves copy its results to pinned host memory and raise its
+ * completion flag.  Ring space and request slots are admitted by capacity
+ * (PA_E_CAPACITY when full); a request's results equal its trace alone.
+ * While the kernel runs the host makes no GPU call (submit and collect
+ * only touch pinned memory); the kernel stays resident, idle waves polling
+ * device memory every ~16 us, until pa_gpu_serve_stop, and submit or poll
+ * launch it when needed.  Other work of the process on the same device may
+ * share its hardware queue: stop the server before it.  particle_capacity
+ * 64 .. 2^24 (68 B of pinned host memory and 20 B of device memory each);
+ * waves_per_cu 0 = 8; idle_polls is reserved (0). */
 pa_status   pa_gpu_serve_create(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
                                 size_t particle_capacity, int32_t request_slots, int32_t waves_per_cu,
                                 int32_t idle_polls, pa_gpu_server** out);

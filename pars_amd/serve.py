@@ -203,7 +203,7 @@ class ResidentServer:
                  request_slots: int = 4096, waves_per_cu: int = 0, memory_fraction: float = 0.02):
         from . import DeviceServer
         self.ctx, self.fields = ctx, list(fields)
-        if capacity is None:  # admission by memory: pinned ring of 68 B per particle (in + out)
+        if capacity is None:  # admission by memory: 68 B of pinned ring per particle (in + out), 20 B on device
             free = ctx.device_info()["free_bytes"]
             capacity = min(int(free * memory_fraction) // 68, 1 << 24)
         self.capacity = capacity
@@ -241,7 +241,9 @@ class ResidentServer:
                 wait = waiting[0].arrival - (clock() - t0)
                 if wait > 0:
                     time.sleep(min(wait, 0.0005))
-        return steps_total, clock() - t0
+        wall = clock() - t0
+        self.server.stop()  # other work of the process may share the kernel's hardware queue
+        return steps_total, wall
 
     def close(self):
         self.server.stop()
