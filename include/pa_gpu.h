@@ -417,6 +417,27 @@ pa_status   pa_gpu_tracer_trace_union_round_async(pa_gpu_tracer* tracer, const p
                                                   int32_t* d_seed_steps, pa_round_result* d_result);
 pa_status   pa_gpu_round_result_check(pa_gpu_ctx* ctx, const pa_round_result* h_result, size_t vertex_capacity,
                                       size_t index_capacity);
+/* Extension: the asynchronous round for a loop of rounds
+ * (pars/source/pipeline.cpp:153-209 traces round after round and uses the
+ * curves once the loop ends).  The trace runs on the context's stream; the
+ * finish (pack, index, colour, result words) runs on a second stream of
+ * the context once the trace is done, so the next round's trace, queued
+ * right after, runs while this round finishes.  Successive calls alternate
+ * two internal workspaces: a call waits (on the device) for the finish of
+ * the call before last.  The caller alternates two sets of round buffers
+ * likewise (slots, counts, continuation, outputs, endpoints, result): the
+ * next call must not touch this call's.  Work queued on the context later
+ * is ordered after this round's trace but not its finish: pa_gpu_round_join
+ * orders the context's stream after every queued finish, and
+ * pa_gpu_synchronize waits for them.  Not while capturing (PA_E_STATE). */
+pa_status   pa_gpu_tracer_trace_union_round_pipelined(pa_gpu_tracer* tracer, const pa_particle* d_particles,
+                                                      size_t n, size_t region_slots, float* d_slots,
+                                                      int32_t* d_vertex_counts, pa_particle* d_continuation,
+                                                      pa_color_mode mode, float parameter, float* d_vertices,
+                                                      size_t vertex_capacity, int32_t* d_indices,
+                                                      size_t index_capacity, float* d_colors, float* d_endpoints,
+                                                      int32_t* d_seed_steps, pa_round_result* d_result);
+pa_status   pa_gpu_round_join(pa_gpu_ctx* ctx);
 /* Counted prune of step-major slots (row stride n seeds) into the pruned
  * particle-major layout; same outputs as pa_gpu_prune_counted. */
 pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slots, size_t n,

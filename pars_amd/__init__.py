@@ -139,6 +139,10 @@ class Context:
     def synchronize(self):
         self.check(lib().pa_gpu_synchronize(self._h), "synchronize")
 
+    def round_join(self):
+        """Order the context's stream after every pipelined round's finish (pa_gpu_round_join)."""
+        self.check(lib().pa_gpu_round_join(self._h), "round_join")
+
     def close(self):
         if self._h:
             lib().pa_gpu_ctx_destroy(self._h)
@@ -510,6 +514,22 @@ class particle_tracer:
             C.c_void_p(b["out_indices"].ptr), C.c_size_t(vcap), C.c_void_p(b["out_colors"].ptr),
             C.c_void_p(b["endpoints"].ptr), C.c_void_p(b["seed_steps"].ptr), C.c_void_p(result.ptr)),
             "trace_union_round_async")
+
+    def trace_union_round_pipelined(self, particles: Particles, region_slots: int, buffers, result: DeviceBuffer,
+                                    mode: int = 2, parameter: float = 0.75, vertex_capacity: Optional[int] = None):
+        """The asynchronous round whose finish runs on the context's second stream while the next
+        round traces (pa_gpu_tracer_trace_union_round_pipelined).  Alternate two sets of buffers
+        and results; Context.round_join / synchronize order later work after the finishes."""
+        n = particles.n
+        b = buffers
+        vcap = vertex_capacity or max(1, n * region_slots)
+        self.ctx.check(lib().pa_gpu_tracer_trace_union_round_pipelined(
+            self._h, C.c_void_p(particles.ptr), C.c_size_t(n), C.c_size_t(region_slots),
+            C.c_void_p(b["vertices"].ptr), C.c_void_p(b["counts"].ptr), C.c_void_p(b["cont"].ptr), C.c_int(mode),
+            C.c_float(parameter), C.c_void_p(b["out_vertices"].ptr), C.c_size_t(vcap),
+            C.c_void_p(b["out_indices"].ptr), C.c_size_t(vcap), C.c_void_p(b["out_colors"].ptr),
+            C.c_void_p(b["endpoints"].ptr), C.c_void_p(b["seed_steps"].ptr), C.c_void_p(result.ptr)),
+            "trace_union_round_pipelined")
 
     def round_buffers(self, n: int, region_slots: int):
         """Device buffers of one fused round (slots, counts, outputs)."""

@@ -39,6 +39,15 @@ pa_status scratch(pa_gpu_ctx* ctx, size_t bytes, void** out) {
   return PA_OK;
 }
 
+pa_status round_join(pa_gpu_ctx* ctx) {
+  for (auto& r : ctx->round_set)
+    if (r.pending) {
+      PA_HIP(ctx, hipStreamWaitEvent(ctx->stream, r.finished, 0));
+      r.pending = false;
+    }
+  return PA_OK;
+}
+
 static hipEvent_t take_event(pa_gpu_ctx* ctx) {
   if (!ctx->event_pool.empty()) {
     hipEvent_t e = ctx->event_pool.back();
@@ -142,7 +151,15 @@ pa_status pa_gpu_ctx_destroy(pa_gpu_ctx* ctx) {
   if (!ctx) return PA_E_INVALID;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->finish_stream) (void)hipStreamSynchronize(ctx->finish_stream);
   drain(ctx, true);
+  for (auto& r : ctx->round_set) {
+    if (r.traced) (void)hipEventDestroy(r.traced);
+    if (r.finished) (void)hipEventDestroy(r.finished);
+    if (r.ws) (void)hipFree(r.ws);
+    if (r.counters) (void)hipFree(r.counters);
+  }
+  if (ctx->finish_stream) (void)hipStreamDestroy(ctx->finish_stream);
   // teardown: failures here have nowhere to go
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
@@ -158,6 +175,7 @@ const char* pa_gpu_last_error(const pa_gpu_ctx* ctx) { return ctx ? ctx->error.c
 pa_status pa_gpu_synchronize(pa_gpu_ctx* ctx) {
   if (!ctx) return PA_E_INVALID;
   PA_TRY(capture_guard(ctx));
+  PA_TRY(round_join(ctx));
   PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return PA_OK;
 }

@@ -92,6 +92,18 @@ struct pa_gpu_ctx {
   void* static_scratch = nullptr;  // the device's fixed no-allocation workspace (never freed)
   bool capturing = false;          // between pa_gpu_graph_begin and pa_gpu_graph_end
   bool capture_failed = false;     // a call that reads back was made while capturing
+  // Pipelined rounds: a round's finish runs on finish_stream while the next round traces on stream;
+  // two sets of workspace and counters alternate, each free again once its finish event has passed.
+  hipStream_t finish_stream = nullptr;
+  struct RoundSet {
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    void* counters = nullptr;  // 4 KiB, used in place of the context's counters
+    hipEvent_t traced = nullptr, finished = nullptr;
+    bool pending = false;      // a finish was queued and the context's stream has not waited for it
+  };
+  RoundSet round_set[2];
+  uint64_t pipelined_rounds = 0;
 };
 
 struct pa_gpu_tracer {
@@ -177,14 +189,18 @@ pa_status finish_group8(pa_gpu_ctx* ctx, const float* d_slots, int64_t n, int64_
 size_t finish_ws_bytes(int64_t n);
 // The fused round (trace on the hybrid engine + prune + index + colour)
 // with the caller's device workspace of union_round_ws_bytes(n) bytes; the
-// rest is pa_gpu_tracer_trace_union_round.
+// rest is pa_gpu_tracer_trace_union_round.  With finish_stream (and
+// d_result), the finish waits for the trace through the event traced and
+// runs on finish_stream; the trace stays on the context's stream.
 size_t union_round_ws_bytes(int64_t n);
 pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n, size_t region_slots,
                       float* d_slots, int32_t* d_vertex_counts, pa_particle* d_continuation, size_t* n_continuation,
                       pa_color_mode mode, float parameter, float* d_vertices, size_t vertex_capacity,
                       size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
                       float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws,
-                      pa_round_result* d_result);
+                      pa_round_result* d_result, hipStream_t finish_stream = nullptr, hipEvent_t traced = nullptr);
+// Make the context's stream wait for every pipelined round's finish.
+pa_status round_join(pa_gpu_ctx* ctx);
 // Fixed workspace of the no-allocation fused round (noalloc.hip): its
 // device address on the context's device, looked up once per device at
 // context creation.

@@ -914,6 +914,61 @@ pa_status pa_gpu_tracer_trace_union_round_async(pa_gpu_tracer* t, const pa_parti
                      d_endpoints, d_seed_steps, nullptr, ws, d_result);
 }
 
+pa_status pa_gpu_tracer_trace_union_round_pipelined(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n,
+                                                    size_t region_slots, float* d_slots, int32_t* d_vertex_counts,
+                                                    pa_particle* d_continuation, pa_color_mode mode, float parameter,
+                                                    float* d_vertices, size_t vertex_capacity, int32_t* d_indices,
+                                                    size_t index_capacity, float* d_colors, float* d_endpoints,
+                                                    int32_t* d_seed_steps, pa_round_result* d_result) {
+  if (!t || !d_result) return PA_E_INVALID;
+  pa_gpu_ctx* ctx = t->ctx;
+  if (ctx->capturing) return fail(ctx, PA_E_STATE, "pipelined rounds run on two streams: not while capturing");
+  if (n > size_t(INT32_MAX)) return fail(ctx, PA_E_INVALID, "round: at most 2^31-1 seeds per call");
+  if (!ctx->finish_stream) {
+    PA_HIP(ctx, hipStreamCreateWithFlags(&ctx->finish_stream, hipStreamNonBlocking));
+    for (auto& r : ctx->round_set) {
+      PA_HIP(ctx, hipEventCreateWithFlags(&r.traced, hipEventDisableTiming));
+      PA_HIP(ctx, hipEventCreateWithFlags(&r.finished, hipEventDisableTiming));
+      PA_HIP(ctx, hipMalloc(&r.counters, 4096));
+    }
+  }
+  auto& R = ctx->round_set[ctx->pipelined_rounds & 1];
+  // the round before last used this set: its finish must be done before this trace reuses the
+  // workspace and counters (and, by the caller's alternation, the buffers)
+  if (R.pending) PA_HIP(ctx, hipStreamWaitEvent(ctx->stream, R.finished, 0));
+  R.pending = false;
+  const size_t need = n ? union_round_ws_bytes(int64_t(n)) : 0;
+  if (need > R.ws_bytes) {
+    PA_HIP(ctx, hipEventSynchronize(R.finished));
+    if (R.ws) PA_HIP(ctx, hipFree(R.ws));
+    R.ws = nullptr;
+    R.ws_bytes = 0;
+    PA_HIP(ctx, hipMalloc(&R.ws, need + need / 4));
+    R.ws_bytes = need + need / 4;
+  }
+  struct CountersRestore {
+    pa_gpu_ctx* c;
+    void* p;
+    ~CountersRestore() { c->counters = p; }
+  } restore{ctx, ctx->counters};
+  ctx->counters = R.counters;
+  PA_TRY(union_round(t, d_particles, n, region_slots, d_slots, d_vertex_counts, d_continuation, nullptr, mode,
+                     parameter, d_vertices, vertex_capacity, nullptr, d_indices, index_capacity, nullptr, d_colors,
+                     d_endpoints, d_seed_steps, nullptr, n ? R.ws : nullptr, d_result,
+                     n ? ctx->finish_stream : nullptr, R.traced));
+  if (n) {
+    PA_HIP(ctx, hipEventRecord(R.finished, ctx->finish_stream));
+    R.pending = true;
+  }
+  ++ctx->pipelined_rounds;
+  return PA_OK;
+}
+
+pa_status pa_gpu_round_join(pa_gpu_ctx* ctx) {
+  if (!ctx) return PA_E_INVALID;
+  return round_join(ctx);
+}
+
 pa_status pa_gpu_round_result_check(pa_gpu_ctx* ctx, const pa_round_result* r, size_t vertex_capacity,
                                     size_t index_capacity) {
   if (!ctx || !r) return PA_E_INVALID;
@@ -947,8 +1002,9 @@ pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n
                       pa_color_mode mode, float parameter, float* d_vertices, size_t vertex_capacity,
                       size_t* n_vertices, int32_t* d_indices, size_t index_capacity, size_t* n_indices,
                       float* d_colors, float* d_endpoints, int32_t* d_seed_steps, uint64_t* total_steps, void* ws,
-                      pa_round_result* d_result) {
+                      pa_round_result* d_result, hipStream_t finish_stream, hipEvent_t traced) {
   if (!t || (!d_result && (!n_vertices || !n_indices))) return PA_E_INVALID;
+  if (finish_stream && (!d_result || !traced)) return PA_E_INVALID;
   pa_gpu_ctx* ctx = t->ctx;
   if (n_vertices) *n_vertices = 0;
   if (n_indices) *n_indices = 0;
@@ -979,6 +1035,18 @@ pa_status union_round(pa_gpu_tracer* t, const pa_particle* d_particles, size_t n
                           d_endpoints, d_seed_steps, total_steps, pairs, true));
   char* cbase = static_cast<char*>(ctx->counters);
   auto* totals = reinterpret_cast<int64_t*>(cbase + 1056);
+  // A pipelined round finishes on its own stream once the trace is done; the context's stream is free
+  // for the next round's trace meanwhile.  Every launch below goes to the finish stream.
+  struct StreamRestore {
+    pa_gpu_ctx* c;
+    hipStream_t s;
+    ~StreamRestore() { c->stream = s; }
+  } restore{ctx, ctx->stream};
+  if (finish_stream) {
+    PA_HIP(ctx, hipEventRecord(traced, ctx->stream));
+    PA_HIP(ctx, hipStreamWaitEvent(finish_stream, traced, 0));
+    ctx->stream = finish_stream;
+  }
   PA_TRY(finish_group8(ctx, d_slots, int64_t(n), int64_t(region_slots), d_vertex_counts, pairs, int(mode), parameter,
                        d_vertices, vertex_capacity, d_indices, index_capacity, d_colors,
                        static_cast<char*>(ws) + pair_bytes, totals, reinterpret_cast<unsigned*>(cbase + 1036)));
