@@ -284,6 +284,21 @@ def endpoint_errors(pa, orc, p, share, storage, interp, sample_stride):
                 max_endpoint_err=float(d.max()) if len(d) else None,
                 rms_endpoint_err=float(np.sqrt((d ** 2).mean())) if len(d) else None,
                 max_err_over_block_extent=float(d.max() / extent) if len(d) else None,
+                p99_endpoint_err=float(np.percentile(d, 99)) if len(d) else None,
+                _ends=ends[sel].copy(), _steps=ssteps[sel].copy())
+
+
+def storage_error(e16, e32, extent):
+    """A 16-bit form's sampled endpoints against its FP32 twin on the same
+    interpolant (seeds with the same step count): the storage part of the
+    error, apart from the FP32 arithmetic floor against FP64."""
+    same = e16["_steps"] == e32["_steps"]
+    d = np.linalg.norm(e16["_ends"][same, :3].astype(np.float64) - e32["_ends"][same, :3].astype(np.float64),
+                       axis=1)
+    return dict(sample=int(len(same)), same_step_count_frac=float(same.mean()),
+                max_endpoint_err=float(d.max()) if len(d) else None,
+                rms_endpoint_err=float(np.sqrt((d ** 2).mean())) if len(d) else None,
+                max_err_over_block_extent=float(d.max() / extent) if len(d) else None,
                 p99_endpoint_err=float(np.percentile(d, 99)) if len(d) else None)
 
 
@@ -373,7 +388,7 @@ def bf16_ratio(forms, extra):
 
 
 FTLE_FORMS = (("fp32-cells", "FP32", "cells"), ("fp32", "FP32", "nodes"), ("bf16-cells", "BF16", "cells"),
-              ("fp16-cells", "FP16", "cells"), ("bf16", "BF16", "nodes"))
+              ("fp16-cells", "FP16", "cells"), ("bf16", "BF16", "nodes"), ("fp32-cells-consistent", "FP32", "cells"))
 
 
 def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01, forms=FTLE_FORMS):
@@ -388,9 +403,10 @@ def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01, forms=FTLE_FORMS):
     against an FP64 flow map of the analytic field (the oracle's FP64 twin,
     consistent interpolation), relative to max(|FTLE|, 1e-3); with seeds on
     the nodes, the reference's interpolation defect (a quotient that rounds
-    up to the node reads the next node) dominates it for every storage form.
-    The storage error alone is the FTLE of a 16-bit form against the FP32
-    form's on the same sub-cube."""
+    up to the node reads the next node) dominates it for every storage form
+    with the reference weights; the consistent-weight FP32 form shows the
+    FP32 arithmetic floor alone.  The storage error alone is the FTLE of a
+    16-bit form against the FP32 form's on the same sub-cube."""
     from pars_amd import workloads as W
 
     h = 2.0 * np.pi / (n - 1)
@@ -421,6 +437,9 @@ def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01, forms=FTLE_FORMS):
             field.set_layout(pa.LAYOUT_CELLS)
         fm = pa.flow_map_generator(ctx)
         fm.set_vector_field(field)
+        consistent = "consistent" in name.split("-")
+        interp = pa.INTERP_CONSISTENT if consistent else pa.INTERP_REFERENCE
+        fm.tracer.set_interpolation(interp)
         fm.set_integrator(pa.RUNGE_KUTTA_4)
         fm.set_step_size(dt)
         out, dims, spacing, _ = fm.generate((n, n, n), steps + 1, 1.0)  # warm-up
@@ -462,7 +481,8 @@ def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01, forms=FTLE_FORMS):
             p["remaining_iterations"] = steps + 1
             p["vector_field_index"] = -1
             of = orc.Field(field.read(), np.zeros(3, np.float32), size, sp)
-            q, _, _, _ = orc.advect(p, of, dt=dt, integrator=orc.RK4)
+            q, _, _, _ = orc.advect(p, of, dt=dt, integrator=orc.RK4,
+                                    interp=orc.INTERP_CONSISTENT if consistent else orc.INTERP_REFERENCE)
             got = flow[ij[:, 0], ij[:, 1], ij[:, 2]]
             res["flow_map_sample_bitexact"] = bool(got.tobytes() == q["position"][:, :3].copy().tobytes())
             # FP64 post-processing of the same flow-map values
@@ -474,13 +494,15 @@ def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01, forms=FTLE_FORMS):
             rel = np.abs(gsub - truth) / np.maximum(np.abs(truth), 1e-3)
             res["ftle_err_vs_fp64"] = dict(max_rel=float(rel.max()), rms_rel=float(np.sqrt((rel ** 2).mean())),
                                            p99_rel=float(np.percentile(rel, 99)), nodes=int(rel.size))
-            if fp32_sub is None and st == "FP32":
+            if fp32_sub is None and st == "FP32" and not consistent:
                 fp32_sub = gsub
             elif fp32_sub is not None:
                 rel = np.abs(gsub - fp32_sub) / np.maximum(np.abs(fp32_sub), 1e-3)
-                res["ftle_err_vs_fp32_form"] = dict(max_rel=float(rel.max()),
-                                                    rms_rel=float(np.sqrt((rel ** 2).mean())),
-                                                    p99_rel=float(np.percentile(rel, 99)))
+                # a 16-bit form against the FP32 one: the storage error; the consistent form against it:
+                # what the interpolation defect adds
+                key = "ftle_diff_vs_fp32_reference_interp" if consistent else "ftle_err_vs_fp32_form"
+                res[key] = dict(max_rel=float(rel.max()), rms_rel=float(np.sqrt((rel ** 2).mean())),
+                                p99_rel=float(np.percentile(rel, 99)))
         field.release()
         out_forms[name] = res
     head = dict(out_forms[forms[0][0]])
@@ -724,7 +746,22 @@ def main():
             p, st, interp = pipes[form]
             t0 = time.perf_counter()
             extra[form]["fp64_error"] = endpoint_errors(pa, orc, p, share, st, interp, 32)
-            log("errors %s (%.1f s): %s" % (form, time.perf_counter() - t0, extra[form]["fp64_error"]))
+            log("errors %s (%.1f s): %s" % (form, time.perf_counter() - t0,
+                                            {k: v for k, v in extra[form]["fp64_error"].items() if k[0] != "_"}))
+        # the split of a 16-bit form's error: its FP32 twin (same layout, interpolant and engine) against
+        # FP64 is the arithmetic floor; the 16-bit form against that twin is the storage part
+        for form in forms:
+            if storage_of[form] == pa.FP32:
+                continue
+            twin = "fp32" + form[form.index("-"):] if "-" in form else "fp32"
+            if twin in extra and "fp64_error" in extra[twin]:
+                extra[form]["storage_error_vs_fp32"] = dict(
+                    twin=twin, **storage_error(extra[form]["fp64_error"], extra[twin]["fp64_error"],
+                                               float(share.local.size[0])))
+                log("storage error %s vs %s: %s" % (form, twin, extra[form]["storage_error_vs_fp32"]))
+        for form in forms:
+            for k in ("_ends", "_steps"):
+                extra[form]["fp64_error"].pop(k, None)
     cpu, parity = None, None
     if not args.no_cpu_baseline:
         # the headline pass once more, kept for the parity check against the oracle
