@@ -549,6 +549,21 @@ def bench_batched(pa, ctx, cache=None):
     gsteps = int(res.download(np.uint64, 2)[0])
     graph_same = bool(gsteps == steps and pool.position.download(np.float32, 4 * wl.n).tobytes() == want_pos.tobytes())
     g.close()
+    # the same batch with every set's curves: bounded passes write each particle's slots into an arena,
+    # and each set is finished (prune, index, colour, in the reference's chunks) as soon as it has ended
+    ctx.timing_reset()
+    ctx.timing(True)
+    vt = B.run_batch_vertices(ctx, wl, fields, tun.waves_per_cu)
+    ctx.synchronize()
+    ctx.timing(False)
+    vk = ctx.kernel_time("pool_run")
+    pass_ms = vk[0]
+    curves = dict(vt, steps_per_s=vt["steps"] / vt["seconds"], pool_run_ms=pass_ms,
+                  pool_run_roofline=dict(algorithmic_bytes_per_step=48 + 16,
+                                         achieved_gbs=vt["steps"] * 64 / (pass_ms * 1e-3) / 1e9,
+                                         frac=vt["steps"] * 64 / (pass_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
+                  note="vertices of every set kept and finished on the device set by set (counted prune, "
+                       "index, colour per chunk of <= 134,217,727 vertices), then discarded")
     for f in fields:
         f.release()
     gather = 48  # BF16: 8 corners x 3 components x 2 B
@@ -561,6 +576,7 @@ def bench_batched(pa, ctx, cache=None):
                               algorithmic_bytes_per_step=gather, achieved_gbs=steps * gather / (kernel_ms * 1e-3) / 1e9,
                               peak=HBM_PEAK_GBS, frac=steps * gather / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
                 graph=dict(seconds_per_round=gsec, steps_per_s=gsteps / gsec, same_results=graph_same),
+                curves=curves,
                 tuning=dict(layout="cells" if tun.layout == pa.LAYOUT_CELLS else "nodes",
                             waves_per_cu=tun.waves_per_cu, sample_rates=tun.table))
 
@@ -778,6 +794,12 @@ def main():
         del gpu, ofields
         log("cpu baseline (%.1f s): %s" % (time.perf_counter() - t0, cpu))
         log("parity of the headline pass against the oracle:", parity)
+
+    # the forms' buffers (two sets for a pipelined form, ~14 GB a set) are not needed past this point
+    pipes.clear()
+    p0 = None
+    import gc
+    gc.collect()
 
     workloads = {}
     if not args.no_workloads:

@@ -492,6 +492,22 @@ pa_status   pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, si
 pa_status   pa_gpu_pool_run(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
                             const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
                             uint64_t* total_steps);
+/* pa_gpu_pool_run that also keeps each entry's curve: entry i writes the
+ * slots of its one-block reference round (particle_tracer.cpp:243-326) to
+ * d_slots (vec4) from slot d_slot_base[i] on: the seed when it is first
+ * stepped, each stepped vertex, and the termination vertex (none for
+ * remaining_iterations < 2), and on its end its vertex count (seed + steps
+ * + termination) to d_vertex_counts[i].  Slots past the count are not
+ * written.  With a seed set's entries consecutive and its bases i * R, the
+ * set's slots are the reference's chunk layout for that set, ready for
+ * pa_gpu_prune_counted / index / colour once the set has ended (its request
+ * counter).  An entry whose slots [base, base + max(R, 1)) do not lie in
+ * [0, slot_capacity) becomes PA_POOL_INVALID like a bad field index.
+ * Bounded passes (max_iterations) keep writing where they left off. */
+pa_status   pa_gpu_pool_run_vertices(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
+                                     const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
+                                     const int64_t* d_slot_base, float* d_slots, size_t slot_capacity,
+                                     int32_t* d_vertex_counts, uint64_t* total_steps);
 
 /* Asynchronous forms of pa_gpu_pool_init and pa_gpu_pool_run (capturable):
  * the rejected-entry count (uint32) and the (steps, rejected) pair (2 x

@@ -670,6 +670,19 @@ class ParticlePool:
                                              C.c_int32(waves_per_cu), C.byref(steps)), "pool_run")
         return int(steps.value)
 
+    def run_vertices(self, tracer: "particle_tracer", fields, slot_base: DeviceBuffer, slots: DeviceBuffer,
+                     counts: DeviceBuffer, n: int = None, max_iterations: int = 0, waves_per_cu: int = 0):
+        """run() that also writes each entry's one-block curve slots (pa_gpu_pool_run_vertices):
+        slot_base int64 per entry, slots vec4, counts int32 per entry."""
+        arr = (C.c_void_p * len(fields))(*[f.handle for f in fields])
+        steps = C.c_uint64()
+        pc = self._c()
+        self.ctx.check(lib().pa_gpu_pool_run_vertices(
+            tracer._h, arr, C.c_int32(len(fields)), C.byref(pc), C.c_size_t(self.n if n is None else n),
+            C.c_int32(max_iterations), C.c_int32(waves_per_cu), C.c_void_p(slot_base.ptr), C.c_void_p(slots.ptr),
+            C.c_size_t(slots.nbytes // 16), C.c_void_p(counts.ptr), C.byref(steps)), "pool_run_vertices")
+        return int(steps.value)
+
     def results(self, n: int = None):
         n = self.n if n is None else n
         return dict(position=self.position.download(np.float32, 4 * n).reshape(n, 4),

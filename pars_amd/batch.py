@@ -186,3 +186,174 @@ def run_batch(ctx: Context, wl: BatchWorkload, fields: List[VectorField], waves_
     t0 = time.perf_counter()
     steps = pool.run(t, fields, waves_per_cu=waves_per_cu)
     return steps, time.perf_counter() - t0, pool
+
+
+# ---------------------------------------------------------------- curves of the sets
+
+CHUNK_VERTICES = 134_217_727  # vertices per chunk of pa::integral_curves (particle_tracer.cpp:216-222)
+
+
+def set_order(wl: BatchWorkload) -> np.ndarray:
+    """Permutation of wl.particles that keeps each seed set contiguous, sets in
+    set order, a set's seeds in locality order: the order a set's curves come
+    out in (neighbouring lanes still gather neighbouring cells)."""
+    sid = wl.particles["original_rank"].astype(np.int64)
+    within = np.empty(len(sid), np.int64)
+    loc = locality_order(wl.particles, wl.h)
+    within[loc] = np.arange(len(loc))
+    return np.lexsort((within, sid))
+
+
+class _Arena:
+    """First-fit allocator of slot ranges in the vertex arena."""
+
+    def __init__(self, slots: int):
+        self.free = [(0, int(slots))]  # (start, length), sorted by start
+
+    def take(self, n: int) -> Optional[int]:
+        for k, (s, l) in enumerate(self.free):
+            if l >= n:
+                self.free[k] = (s + n, l - n)
+                if self.free[k][1] == 0:
+                    del self.free[k]
+                return s
+        return None
+
+    def give(self, s: int, n: int):
+        self.free.append((s, n))
+        self.free.sort()
+        merged = []
+        for a, l in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == a:
+                merged[-1] = (merged[-1][0], merged[-1][1] + l)
+            else:
+                merged.append((a, l))
+        self.free = merged
+
+
+@dataclass
+class CurveChunk:
+    """One chunk of a finished set's curves, valid during the consumer call:
+    device pointers and counts of its vertices (vec4), indices (int32, local
+    to the chunk) and colours (vec4); seeds [seed_begin, seed_end) of the set."""
+    set_id: int
+    chunk: int
+    seed_begin: int
+    seed_end: int
+    vertices: int
+    n_vertices: int
+    indices: int
+    n_indices: int
+    colors: int
+
+
+def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField], waves_per_cu: int = 0,
+                       arena_slots: Optional[int] = None, pass_iterations: int = 400, consumer=None,
+                       color_mode: int = 2, color_parameter: float = 0.75, chunk_vertices: int = CHUNK_VERTICES,
+                       order: Optional[np.ndarray] = None):
+    """The batch with every set's curves, streamed out set by set.
+
+    Sets are admitted in set order while their slots (size x R) fit the
+    vertex arena; the pool advances every admitted particle by at most
+    pass_iterations per pass (pa_gpu_pool_run_vertices: each particle writes
+    its one-block reference slots into its set's part of the arena).  After
+    each pass, a set whose particles have all ended is finished on the device
+    in chunks of at most chunk_vertices slots, as the reference chunks its
+    curves (chunk = chunk_vertices // R seeds): counted prune, index, colour;
+    each chunk goes to consumer(CurveChunk), then the set's arena range takes
+    the next sets.  Returns totals (steps, vertices, indices, sets, passes,
+    seconds)."""
+    from . import lib, C  # noqa: F401  (ctypes entry points)
+    order = set_order(wl) if order is None else order
+    parts = wl.particles[order]
+    sizes = np.array([s["size"] for s in wl.sets], np.int64)
+    Rs = np.array([s["steps"] + 2 for s in wl.sets], np.int64)
+    pool_off = np.concatenate([[0], np.cumsum(sizes)])
+    sid = parts["original_rank"].astype(np.int64)
+    if not np.array_equal(sid, np.repeat(np.arange(len(sizes)), sizes)):
+        raise ValueError("run_batch_vertices: the particles must be set by set (set_order)")
+    need = sizes * Rs
+    if arena_slots is None:
+        arena_slots = int(max(need.max(), min(need.sum(), 1 << 31)))
+    if need.max() > arena_slots:
+        raise ValueError("a set needs %d slots, the arena holds %d" % (need.max(), arena_slots))
+    if chunk_vertices < Rs.max():
+        raise ValueError("chunk_vertices below a set's slots per seed")
+    n = len(parts)
+    t = make_tracer(ctx)
+    dev = Particles.from_host(ctx, parts)
+    pool = ParticlePool(ctx, n, requests=len(wl.sets))
+    base = ctx.alloc(max(n, 1) * 8)
+    counts = ctx.alloc(max(n, 1) * 4)
+    arena = ctx.alloc(int(arena_slots) * 16)
+    cap = int(min(chunk_vertices, max(need.max(), 1)))
+    vbuf, ibuf, cbuf = ctx.alloc(cap * 16), ctx.alloc(cap * 4), ctx.alloc(cap * 16)
+    offs = ctx.alloc((int(sizes.max()) + 1) * 8)
+    alloc = _Arena(arena_slots)
+    where = {}           # admitted set -> arena slot offset
+    finished = set()
+    next_set = 0
+    n_hi = 0
+    tot = dict(steps=0, vertices=0, indices=0, sets=0, passes=0, chunks=0)
+    L = lib()
+    t0 = time.perf_counter()
+
+    def admit():
+        nonlocal next_set, n_hi
+        while next_set < len(sizes):
+            s = next_set
+            off = alloc.take(int(need[s]))
+            if off is None:
+                return
+            where[s] = off
+            a, b = int(pool_off[s]), int(pool_off[s + 1])
+            base.upload(off + np.arange(b - a, dtype=np.int64) * Rs[s], byte_offset=a * 8)
+            pc = pool._c(a)
+            st = L.pa_gpu_pool_init(ctx.handle, C.c_void_p(dev.buf.at(48 * a)), C.c_size_t(b - a),
+                                    C.c_int32(len(fields)), C.byref(pc))
+            ctx.check(st, "pool_init")
+            n_hi = max(n_hi, b)
+            pool.n = max(pool.n, b)
+            next_set += 1
+
+    def finish(s):
+        off, R = where.pop(s), int(Rs[s])
+        a = int(pool_off[s])
+        per = max(1, chunk_vertices // R)
+        for k, j0 in enumerate(range(0, int(sizes[s]), per)):
+            j1 = min(int(sizes[s]), j0 + per)
+            m, ni = C.c_size_t(), C.c_size_t()
+            ctx.check(L.pa_gpu_prune_counted(ctx.handle, C.c_void_p(arena.at((off + j0 * R) * 16)), C.c_size_t(R),
+                                             C.c_void_p(counts.at((a + j0) * 4)), C.c_size_t(j1 - j0),
+                                             C.c_void_p(vbuf.ptr), C.c_size_t(cap), C.byref(m), C.c_void_p(offs.ptr)),
+                      "prune_counted")
+            ctx.check(L.pa_gpu_index_generate(ctx.handle, C.c_void_p(vbuf.ptr), m, C.c_void_p(ibuf.ptr),
+                                              C.byref(ni)), "index")
+            ctx.check(L.pa_gpu_color_generate(ctx.handle, C.c_void_p(vbuf.ptr), m, C.c_void_p(ibuf.ptr), ni,
+                                              C.c_int(color_mode), C.c_float(color_parameter),
+                                              C.c_void_p(cbuf.ptr)), "color")
+            tot["vertices"] += m.value
+            tot["indices"] += ni.value
+            tot["chunks"] += 1
+            if consumer is not None:
+                consumer(CurveChunk(s, k, j0, j1, vbuf.ptr, m.value, ibuf.ptr, ni.value, cbuf.ptr))
+        alloc.give(off, int(need[s]))
+        finished.add(s)
+        tot["sets"] += 1
+
+    admit()
+    while len(finished) < len(sizes):
+        if not where:
+            raise RuntimeError("run_batch_vertices: nothing admitted")
+        tot["steps"] += pool.run_vertices(t, fields, base, arena, counts, n=n_hi, max_iterations=pass_iterations,
+                                          waves_per_cu=waves_per_cu)
+        tot["passes"] += 1
+        done = pool.request_done.download(np.int32, len(sizes))
+        for s in sorted(where):
+            if done[s] == sizes[s]:
+                finish(s)
+        admit()
+    ctx.synchronize()
+    tot["seconds"] = time.perf_counter() - t0
+    tot["arena_slots"] = int(arena_slots)
+    return tot

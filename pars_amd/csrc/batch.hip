@@ -5,8 +5,12 @@
 // request at a time); every particle follows the reference round of
 // particle_tracer.cpp:265-326 on its own field alone: it = 1 .. R-1, stop on
 // leaving the field, on the stop rule or at it == R-1, so its endpoint and
-// step count are those of a one-block reference trace (the vertices are not
-// kept).  A particle's result does not depend on what else is in the pool.
+// step count are those of a one-block reference trace.  With vertex output
+// (pa_gpu_pool_run_vertices) each entry also writes that trace's slots: the
+// seed at its first slot, each stepped vertex, the termination vertex, and
+// its vertex count, as the reference round writes them for the particle
+// alone (particle_tracer.cpp:243-326).  A particle's result does not depend
+// on what else is in the pool.
 #include <cstring>
 #include <string>
 
@@ -22,6 +26,14 @@ struct PoolConfig {
   TraceConfig k;                // integrator constants, stop rule, interpolation
   FieldDesc f[kMaxFields];      // field i is particle.vector_field_index == i
   int32_t n_fields;
+};
+
+// Optional vertex output: entry i owns slots [base[i], base[i] + max(R, 1)) of `slots`.
+struct PoolVertices {
+  float4* slots = nullptr;       // null: endpoints only
+  const int64_t* base = nullptr;
+  int32_t* count = nullptr;
+  int64_t capacity = 0;          // slots
 };
 
 __global__ __launch_bounds__(kPoolBlock) void k_pool_init(const pa_particle* __restrict__ P, int64_t n, int n_fields,
@@ -64,10 +76,11 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
                                                          int64_t n, int budget, int rule,
                                                          unsigned long long* __restrict__ work,
                                                          unsigned long long* __restrict__ steps_out,
-                                                         unsigned long long* __restrict__ rejected) {
+                                                         unsigned long long* __restrict__ rejected,
+                                                         const PoolVertices V) {
   const int lane = threadIdx.x & 63;
   bool have = false, open = true;
-  int64_t i = 0;
+  int64_t i = 0, vb = 0;
   float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   int it = 0, R = 0, st = 0, left = 0;
   FieldDesc f = cfg.f[0];
@@ -86,19 +99,24 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
         const unsigned long long idx = base + uint32_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
         if (idx < static_cast<unsigned long long>(n) && status_a[idx] == PA_POOL_ACTIVE) {
           const int32_t fi = field_a[idx];
-          if (fi < 0 || fi >= cfg.n_fields) {
-            // a field this launch was not given: never stepped, never read
+          const int32_t r_idx = rem[idx];
+          const int64_t b_idx = V.slots ? V.base[idx] : 0;
+          if (fi < 0 || fi >= cfg.n_fields ||
+              (V.slots && (b_idx < 0 || b_idx > V.capacity - (r_idx > 1 ? r_idx : 1)))) {
+            // a field this launch was not given, or slots outside the vertex buffer: never stepped
             status_a[idx] = PA_POOL_INVALID;
             atomicAdd(rejected, 1ull);
           } else {
             i = int64_t(idx);
             p = pos[i];
             it = it_a[i];
-            R = rem[i];
+            R = r_idx;
             st = steps_a[i];
             f = cfg.f[fi];
             left = budget;
             have = true;
+            vb = b_idx;
+            if (V.slots && it == 1) V.slots[vb] = p;  // first load: the seed (initialize, :243-252)
           }
         }
       }
@@ -114,6 +132,7 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
         if (advance_rt<S, MODE, INTERP>(cfg.k, f, p, next, rule) != 0) {
           fin = true;
         } else {
+          if (V.slots) V.slots[vb + it] = next;
           p = next;
           ++it;
           ++st;
@@ -124,6 +143,11 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
         pos[i] = p;
         it_a[i] = it;
         steps_a[i] = st;
+        if (fin && V.slots) {
+          // the termination vertex at the last slot reached (none for R < 2: the seed alone)
+          if (R >= 2) V.slots[vb + it] = make_float4(-2.0f, -2.0f, -2.0f, 0.0f);
+          V.count[i] = R >= 2 ? it + 1 : 1;
+        }
         if (fin) {
           status_a[i] = PA_POOL_DONE;
           if (request && request_done) {
@@ -213,7 +237,7 @@ __global__ void k_pool_result(const unsigned long long* __restrict__ c, unsigned
 // Enqueues the persistent pool kernel; its step count and rejected entries
 // are left in the context counters 2056 and 2064.
 pa_status pool_launch(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32_t n_fields, const pa_pool* pool,
-                      size_t n, int32_t max_iterations, int32_t waves_per_cu) {
+                      size_t n, int32_t max_iterations, int32_t waves_per_cu, const PoolVertices& V = PoolVertices()) {
   pa_gpu_ctx* ctx = t->ctx;
   if (!fields || n_fields < 1 || n_fields > kMaxFields) return fail(ctx, PA_E_INVALID, "pool: 1 to 16 fields");
   if (max_iterations < 0 || waves_per_cu < 0) return fail(ctx, PA_E_INVALID, "pool: negative budget or waves");
@@ -250,7 +274,7 @@ pa_status pool_launch(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32
                        dim3(unsigned(blocks)), dim3(kPoolBlock), 0, ctx->stream, cfg,
                        reinterpret_cast<float4*>(pool->position), pool->iteration, pool->remaining, pool->steps,
                        pool->status, pool->field, pool->request, pool->request_done, int(pool->n_requests),
-                       int64_t(n), budget, t->stop_rule, counters, counters + 1, counters + 2);
+                       int64_t(n), budget, t->stop_rule, counters, counters + 1, counters + 2, V);
   }, cfg.f[0].layout);
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
@@ -276,6 +300,35 @@ pa_status pa_gpu_pool_run(pa_gpu_tracer* t, const pa_gpu_field* const* fields, i
   if (s[1])
     return fail(ctx, PA_E_INVALID,
                 "pool: " + std::to_string(s[1]) + " entries name a field index >= n_fields; marked invalid");
+  return PA_OK;
+}
+
+pa_status pa_gpu_pool_run_vertices(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32_t n_fields,
+                                   const pa_pool* pool, size_t n, int32_t max_iterations, int32_t waves_per_cu,
+                                   const int64_t* d_slot_base, float* d_slots, size_t slot_capacity,
+                                   int32_t* d_vertex_counts, uint64_t* total_steps) {
+  if (!t || !pool) return PA_E_INVALID;
+  pa_gpu_ctx* ctx = t->ctx;
+  if (total_steps) *total_steps = 0;
+  if (n == 0) return PA_OK;
+  if (!d_slot_base || !d_slots || !d_vertex_counts) return PA_E_INVALID;
+  if (slot_capacity > size_t(INT64_MAX)) return fail(ctx, PA_E_INVALID, "pool: slot capacity above 2^63");
+  PoolVertices V;
+  V.slots = reinterpret_cast<float4*>(d_slots);
+  V.base = d_slot_base;
+  V.count = d_vertex_counts;
+  V.capacity = int64_t(slot_capacity);
+  PA_TRY(pool_launch(t, fields, n_fields, pool, n, max_iterations, waves_per_cu, V));
+  auto* counters = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2048);
+  unsigned long long s[2] = {0, 0};
+  PA_TRY(capture_guard(ctx));
+  PA_HIP(ctx, hipMemcpyAsync(s, counters + 1, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (total_steps) *total_steps = s[0];
+  if (s[1])
+    return fail(ctx, PA_E_INVALID,
+                "pool: " + std::to_string(s[1]) +
+                    " entries name a field index >= n_fields or slots outside the vertex buffer; marked invalid");
   return PA_OK;
 }
 
