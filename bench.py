@@ -219,9 +219,28 @@ class Pipeline:
         return dict(particles=n.value, steps=r["total_steps"], vertices=m.value, indices=ni.value)
 
 
+def tune_pipeline(pa, ctx, p):
+    """The fused round's launch constants for this workload (pars_amd.tune), set on the pipeline's tracer."""
+    from pars_amd import tune as T
+    n = C.c_size_t()
+    o, s, st = p.seed_args
+    ctx.check(pa.lib().pa_gpu_seed_generate(ctx.handle, o, s, st, C.c_int32(R), C.c_int32(21),
+                                            C.c_void_p(p.seeds.ptr), C.c_size_t(p.seeds.nbytes // 48),
+                                            C.byref(n)), "seeds")
+    b = dict(p.ub, out_vertices=p.pruned, out_indices=p.indices, out_colors=p.colors)
+    return T.tune_round(ctx, p.tracer, pa.Particles(p.seeds, n.value), p.region, b, ctx.alloc(48))
+
+
 def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, order, engine=None, fused=False,
-             graph=False, pipelined=False):
+             graph=False, pipelined=False, tune=False, launch=None):
     p = Pipeline(pa, ctx, fields, share, storage, interp, n_seeds, order, engine, fused, graph, pipelined)
+    tuning = None
+    if fused and launch is not None:  # given constants (e.g. for a profiled run), no tuning
+        p.tracer.set_launch(launch[0], launch[1])
+        p.tracer.set_xcd_remap(bool(launch[2]))
+        tuning = dict(quad_switch=launch[0], pair_live=launch[1], xcd_remap=launch[2], given=True)
+    elif tune and fused and not graph:
+        tuning = tune_pipeline(pa, ctx, p).as_dict()
     for _ in range(warmup):
         info = p.step()
     ctx.synchronize()
@@ -253,6 +272,7 @@ def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, or
                                           "round_counts", "round_finish")}
     trace_ms = kt["trace_union"][0] / kt["trace_union"][1] if kt["trace_union"][1] else None  # None: graph replays
     return p, dict(wall_s=wall, ms_per_step=1e3 * wall / steps, steps_per_pass=total // steps,
+                   tuning=tuning,
                    pass_latency_ms=dict(p50=1e3 * float(np.percentile(lat, 50)),
                                         p99=1e3 * float(np.percentile(lat, 99)), max=1e3 * max(lat)),
                    value=total / wall, trace_ms=trace_ms, info=info,
@@ -643,6 +663,8 @@ def main():
     ap.add_argument("--no-workloads", action="store_true", help="skip the FTLE, batched and serving workloads")
     ap.add_argument("--workloads", default="ftle,batched,serving", help="which of the other workloads to run")
     ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the fused round's default launch constants")
+    ap.add_argument("--launch", default="", help="quad_switch,pair_live,xcd_remap for the fused round (no tuning)")
     ap.add_argument("--calib-bytes", type=int, default=0, help="also run a plain copy of this size (counter calibration)")
     # one GPU's share of the host is 16 CPUs on the boxes this runs on
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -691,7 +713,8 @@ def main():
         engine = pa.ENGINE_LANE if "lane" in tokens else pa.ENGINE_HYBRID if order == "group8" else None
         fused = engine == pa.ENGINE_HYBRID and "stages" not in tokens
         p, res = run_form(pa, ctx, fields[key], share, st, interp, n_seeds, args.steps, args.warmup, order, engine,
-                          fused, graph="graph" in tokens, pipelined="pipelined" in tokens)
+                          fused, graph="graph" in tokens, pipelined="pipelined" in tokens, tune=not args.no_tune,
+                          launch=[int(x) for x in args.launch.split(",")] if args.launch else None)
         log("%s: %.3f ms/step, %.3e steps/s, trace kernel %s ms, %s" %
             (form, res["ms_per_step"], res["value"], "%.3f" % res["trace_ms"] if res["trace_ms"] else "-",
              res["info"]))
@@ -742,6 +765,7 @@ def main():
         st = storage_of[form]
         tm = r["trace_ms"]
         extra[form] = dict(value=r["value"], ms_per_step=r["ms_per_step"], trace_kernel_ms=tm,
+                           launch_tuning=r["tuning"],
                            kernel=kernel_name(form), pass_latency_ms=r["pass_latency_ms"],
                            trace_steps_per_s=r["steps_per_pass"] / (tm * 1e-3) if tm else None,
                            steps_per_pass=r["steps_per_pass"], stage_ms=r["stage_ms"] if tm else None,

@@ -411,6 +411,11 @@ class particle_tracer:
     def set_xcd_remap(self, enable: bool):
         self.ctx.check(lib().pa_gpu_tracer_set_xcd_remap(self._h, C.c_int32(int(enable))), "set_xcd_remap")
 
+    def set_launch(self, quad_switch: int = 16, pair_live: int = 32):
+        """Hybrid engine launch constants (pa_gpu_tracer_set_launch); results do not depend on them."""
+        self.ctx.check(lib().pa_gpu_tracer_set_launch(self._h, C.c_int32(quad_switch), C.c_int32(pair_live)),
+                       "set_launch")
+
     def set_interpolation(self, interp: int):
         self.ctx.check(lib().pa_gpu_tracer_set_interpolation(self._h, C.c_int(interp)), "set_interpolation")
 

@@ -59,6 +59,8 @@ struct TraceConfig {
   float stop_eps2;
   int32_t interp;
   int32_t xcd_remap;
+  int32_t quad_switch;       // hybrid engine: a wave moves to quads at this many live particles (1..16)
+  int32_t pair_live;         // hybrid engine: FP32 pair gathers while more lanes than this are live
 };
 
 }  // namespace pa_amd
@@ -120,6 +122,8 @@ struct pa_gpu_tracer {
   int32_t engine = PA_ENGINE_LANE;
   int32_t waves = 0;  // resident waves of the cooperative engine (0 = automatic)
   int32_t xcd_remap = 1;
+  int32_t quad_switch = 16;  // launch constants of the hybrid engine (pa_gpu_tracer_set_launch)
+  int32_t pair_live = 32;
 };
 
 namespace pa_amd {

@@ -580,6 +580,8 @@ void fill_step_constants(const pa_gpu_tracer* t, TraceConfig* cfg) {
   cfg->stop_eps2 = t->stop_eps * t->stop_eps;
   cfg->interp = t->interp;
   cfg->xcd_remap = t->xcd_remap;
+  cfg->quad_switch = t->quad_switch;
+  cfg->pair_live = t->pair_live;
 }
 
 int mode_of_integrator(int integrator) { return mode_of(integrator); }
@@ -705,6 +707,15 @@ pa_status pa_gpu_tracer_set_engine(pa_gpu_tracer* t, pa_engine engine, int32_t w
 pa_status pa_gpu_tracer_set_xcd_remap(pa_gpu_tracer* t, int32_t enable) {
   if (!t) return PA_E_INVALID;
   t->xcd_remap = enable ? 1 : 0;
+  return PA_OK;
+}
+
+pa_status pa_gpu_tracer_set_launch(pa_gpu_tracer* t, int32_t quad_switch, int32_t pair_live) {
+  if (!t) return PA_E_INVALID;
+  if (quad_switch < 1 || quad_switch > 16 || pair_live < 0 || pair_live > 64)
+    return fail(t->ctx, PA_E_INVALID, "launch constants: quad_switch 1..16, pair_live 0..64");
+  t->quad_switch = quad_switch;
+  t->pair_live = pair_live;
   return PA_OK;
 }
 

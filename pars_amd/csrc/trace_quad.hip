@@ -535,8 +535,7 @@ __device__ __forceinline__ void pair_finish(const PairGather& P, bool pair, bool
   }
 }
 
-// Pair gathers while more than this many lanes of the wave are live.
-constexpr int kPairLive = 32;
+// Pair gathers while more than cfg.pair_live lanes of the wave are live (32 by default).
 
 __device__ __forceinline__ float4 pick4(bool c, float4 x, float4 y) {  // value select (no lvalue select)
   return make_float4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
@@ -551,6 +550,8 @@ __device__ __forceinline__ float4 pick4(bool c, float4 x, float4 y) {  // value 
 // the survivors move through LDS onto quads (quad mode, one component per
 // lane) and finish there.  Both modes run the same per-iteration segment
 // machine, so the slots are those of the lane engine, bit for bit.
+// The switch happens at cfg.quad_switch live particles (16 by default, at most kQuadSwitch: a wave has
+// 16 quads).
 constexpr int kQuadSwitch = 16;
 // A wave's vertex stores fall in its 8 seed groups: byte offsets up to
 // 1024 * region - 4 (the w component of seed 63's last slot), which with
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   uint64_t live;
   for (;;) {
     live = __ballot(st.state != Q_IDLE);
-    if (__popcll(live) <= kQuadSwitch) break;
+    if (__popcll(live) <= cfg.quad_switch) break;
     // Cell records: advance_rt of a stepping lane, split around the store
     // (a = 0 stepped, next valid; 1 left the block; 2 stop rule).  Every lane
     // issues the gather, so its registers need no merge of paths (whose
@@ -660,14 +661,14 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         m[i] = static_cast<int>(fminf(fmaxf(floorf(q[i]), 0.0f), static_cast<float>(f.dims[i] - 2)));
       uint64_t r = reinterpret_cast<uint64_t>(cell_record<S>(f, m));
       const uint64_t rown = r;
-      // While the wave is full (more than kPairLive live lanes, where FP32
+      // While the wave is full (more than cfg.pair_live live lanes, where FP32
       // records are pair-gathered), a particle still in the cell
       // of its previous step takes that record from LDS and gathers like a
       // lane that does not step (no line of its own): share round R = 52
       // 0.241 -> 0.219 ms; at R = 1002 no change (1.389 -> 1.391 ms), as
       // the LDS round trip lengthens the long trajectories' steps, so fewer
       // live lanes skip it.
-      const bool reuse = kReuse && __popcll(live) > kPairLive && inside && r == rprev;
+      const bool reuse = kReuse && __popcll(live) > cfg.pair_live && inside && r == rprev;
       const uint64_t W0 = __ballot(inside);
       const uint64_t W1 = __ballot(inside && !reuse);
       const uint64_t W = W1 != 0 ? W1 : W0;
@@ -681,7 +682,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       }
       Gather<S> g;
       [[maybe_unused]] PairGather P;
-      const bool pair = __popcll(live) > kPairLive;  // wave-uniform
+      const bool pair = __popcll(live) > cfg.pair_live;  // wave-uniform
       const bool even = (lane & 1) == 0;
       if constexpr (storage_of(S) == PA_STORAGE_FP32) pair_issue(r, pair, even, P);
       else gather_record<S>(reinterpret_cast<const uint4*>(r), g);
@@ -690,7 +691,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       // keep the exchange (which waits for the loads) after the store
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (storage_of(S) == PA_STORAGE_FP32) pair_finish<S>(P, pair, even, g);
-      if (kReuse && __popcll(live) > kPairLive) {
+      if (kReuse && __popcll(live) > cfg.pair_live) {
 #pragma unroll
         for (int k = 0; k < kChunks; ++k) {
           g.r[k] = sel4(reuse, prev_rec[k][threadIdx.x], g.r[k]);
