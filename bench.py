@@ -237,7 +237,7 @@ def run_form(pa, ctx, fields, share, storage, interp, n_seeds, steps, warmup, or
     tuning = None
     if fused and launch is not None:  # given constants (e.g. for a profiled run), no tuning
         p.tracer.set_launch(launch[0], launch[1])
-        p.tracer.set_xcd_remap(bool(launch[2]))
+        p.tracer.set_xcd_remap(int(launch[2]))
         tuning = dict(quad_switch=launch[0], pair_live=launch[1], xcd_remap=launch[2], given=True)
     elif tune and fused and not graph:
         tuning = tune_pipeline(pa, ctx, p).as_dict()

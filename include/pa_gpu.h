@@ -277,10 +277,13 @@ typedef enum pa_engine {
   PA_ENGINE_HYBRID = 3
 } pa_engine;
 pa_status   pa_gpu_tracer_set_engine(pa_gpu_tracer* tracer, pa_engine engine, int32_t waves);
-/* Extension: XCD-aware workgroup order of the lane engine (default on):
- * the 8 XCDs, which take workgroups round-robin, each process a contiguous
- * eighth of the particle list, so particles seeded close together share an
- * L2.  Results are unchanged. */
+/* Extension: XCD-aware workgroup order (default 1).  The 8 XCDs take
+ * workgroups round-robin; with 1 each processes a contiguous eighth of the
+ * particle list, so particles seeded close together share an L2; with 0
+ * the hardware order stays (every XCD gets every part of the list); with
+ * 2..64 each XCD takes runs of that many consecutive workgroups in turn
+ * (locality within a run, balance across the list).  Results are
+ * unchanged. */
 pa_status   pa_gpu_tracer_set_xcd_remap(pa_gpu_tracer* tracer, int32_t enable);
 /* Extension: launch constants of the hybrid engine, for an autotuner: a
  * wave moves its live particles onto quads once at most quad_switch (1..16,

@@ -408,8 +408,9 @@ class particle_tracer:
         engine (0 = automatic)."""
         self.ctx.check(lib().pa_gpu_tracer_set_engine(self._h, C.c_int(engine), C.c_int32(waves)), "set_engine")
 
-    def set_xcd_remap(self, enable: bool):
-        self.ctx.check(lib().pa_gpu_tracer_set_xcd_remap(self._h, C.c_int32(int(enable))), "set_xcd_remap")
+    def set_xcd_remap(self, order):
+        """XCD workgroup order: 0 hardware order, 1 contiguous eighths, 2..64 runs of that many workgroups."""
+        self.ctx.check(lib().pa_gpu_tracer_set_xcd_remap(self._h, C.c_int32(int(order))), "set_xcd_remap")
 
     def set_launch(self, quad_switch: int = 16, pair_live: int = 32):
         """Hybrid engine launch constants (pa_gpu_tracer_set_launch); results do not depend on them."""

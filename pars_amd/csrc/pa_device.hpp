@@ -456,10 +456,24 @@ __device__ __forceinline__ int64_t uniform(int64_t x) {
 // XCD i % 8).  With remap on, XCD x processes the contiguous x-th eighth of
 // the logical workgroups, so particles that start close together share an
 // XCD's L2.  A bijection for any grid size.
+//
+// remap >= 2 keeps the work spread over the XCDs instead: XCD x takes runs of
+// `remap` consecutive logical workgroups, x, x + 8, ... (run-wise round robin),
+// so neighbouring particles share an L2 within a run while every XCD gets a
+// share of every part of the list (seed lists whose long-lived particles sit
+// in one part of the list would otherwise load a few XCDs only).  Workgroups
+// past the last whole round of 8 runs keep their own index (a bijection).
 __device__ __forceinline__ int64_t logical_block(int remap) {
   const int64_t b = blockIdx.x;
   if (!remap) return b;
-  const int64_t nb = gridDim.x, q = nb / 8, r = nb % 8;
+  const int64_t nb = gridDim.x;
+  if (remap >= 2) {
+    const int64_t c = remap, whole = nb / (8 * c) * (8 * c);
+    if (b >= whole) return b;
+    const int64_t x = b % 8, k = b / 8;
+    return ((k / c) * 8 + x) * c + k % c;
+  }
+  const int64_t q = nb / 8, r = nb % 8;
   const int64_t x = b % 8, k = b / 8;
   return x * q + (x < r ? x : r) + k;
 }

@@ -706,7 +706,8 @@ pa_status pa_gpu_tracer_set_engine(pa_gpu_tracer* t, pa_engine engine, int32_t w
 
 pa_status pa_gpu_tracer_set_xcd_remap(pa_gpu_tracer* t, int32_t enable) {
   if (!t) return PA_E_INVALID;
-  t->xcd_remap = enable ? 1 : 0;
+  if (enable < 0 || enable > 64) return fail(t->ctx, PA_E_INVALID, "xcd order: 0, 1 or a run length 2..64");
+  t->xcd_remap = enable;
   return PA_OK;
 }
 

@@ -22,8 +22,8 @@ import numpy as np
 from . import ROUND_RESULT_DTYPE, Context, DeviceBuffer, Particles, particle_tracer
 
 QUAD_SWITCH = (8, 12, 16)
-PAIR_LIVE = (16, 32, 48, 64)
-XCD_REMAP = (0, 1)
+PAIR_LIVE = (16, 32, 48)
+XCD_REMAP = (0, 1, 2, 4, 8)
 
 
 @dataclass
@@ -37,7 +37,7 @@ class RoundTuning:
 
     def apply(self, tracer: particle_tracer):
         tracer.set_launch(self.quad_switch, self.pair_live)
-        tracer.set_xcd_remap(bool(self.xcd_remap))
+        tracer.set_xcd_remap(int(self.xcd_remap))
 
     def as_dict(self):
         return dict(quad_switch=self.quad_switch, pair_live=self.pair_live, xcd_remap=self.xcd_remap, ms=self.ms,
@@ -69,7 +69,7 @@ def tune_round(ctx: Context, tracer: particle_tracer, seeds: Particles, region_s
     host = np.zeros(1, ROUND_RESULT_DTYPE)
     for q, p, x in itertools.product(quad_switch, pair_live, xcd_remap):
         tracer.set_launch(q, p)
-        tracer.set_xcd_remap(bool(x))
+        tracer.set_xcd_remap(int(x))
         times = []
         for r in range(reps + 1):
             ctx.synchronize()

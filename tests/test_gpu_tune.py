@@ -22,9 +22,9 @@ def test_launch_constants_leave_results_unchanged(ctx, orc, storage):
     region = R + 40
     seeds = pa.seed_generator.generate(ctx, loc.offset, loc.size, loc.spacing * f32(2), R, 21)
     outs = []
-    for q, p, x in [(16, 32, 1), (8, 0, 1), (12, 64, 0), (1, 48, 1)]:
+    for q, p, x in [(16, 32, 1), (8, 0, 1), (12, 64, 0), (1, 48, 1), (16, 32, 4), (16, 48, 64)]:
         t.set_launch(q, p)
-        t.set_xcd_remap(bool(x))
+        t.set_xcd_remap(x)
         r = t.trace_union_round(seeds, region, 2, 0.6,
                                 colors=ctx.upload(np.zeros(4 * seeds.n * region, np.float32)))
         nv, ni = r["n_vertices"], r["n_indices"]

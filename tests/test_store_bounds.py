@@ -88,3 +88,34 @@ def test_clamped_record_index_in_range():
         ci = m[:, 0] * cstride0 + m[:, 1] * cstride1 + m[:, 2]
         assert ci.min() >= 0 and ci.max() < n_cells
         assert ci.max() < 2 ** 32  # the kernel forms the index in 32 bits
+
+
+def logical_block(b, nb, remap):
+    """The XCD-aware workgroup order of the tracers (pa_device.hpp), restated."""
+    if not remap:
+        return b
+    if remap >= 2:
+        c = remap
+        whole = nb // (8 * c) * (8 * c)
+        if b >= whole:
+            return b
+        x, k = b % 8, b // 8
+        return ((k // c) * 8 + x) * c + k % c
+    q, r = nb // 8, nb % 8
+    x, k = b % 8, b // 8
+    return x * q + (x if x < r else r) + k
+
+
+def test_xcd_orders_are_bijections():
+    """Every workgroup order maps the grid onto itself one to one (each particle traced once), and a run
+    order gives every XCD (b % 8) runs of consecutive logical workgroups from every part of the list."""
+    for nb in (1, 7, 8, 9, 63, 64, 100, 1024, 1031, 4097):
+        for remap in (0, 1, 2, 3, 4, 8, 16, 64):
+            m = [logical_block(b, nb, remap) for b in range(nb)]
+            assert sorted(m) == list(range(nb)), (nb, remap)
+    nb, c = 1024, 4
+    for x in range(8):
+        mine = sorted(logical_block(b, nb, c) for b in range(x, nb, 8))
+        runs = [mine[i:i + c] for i in range(0, len(mine), c)]
+        assert all(r == list(range(r[0], r[0] + c)) for r in runs)
+        assert mine[0] < nb // 8 and mine[-1] >= nb - nb // 8  # spread over the whole list
