@@ -248,16 +248,19 @@ class CurveChunk:
 
 
 def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField], waves_per_cu: int = 0,
-                       arena_slots: Optional[int] = None, pass_iterations: int = 400, consumer=None,
+                       arena_slots: Optional[int] = None, pass_iterations: int = 0, consumer=None,
                        color_mode: int = 2, color_parameter: float = 0.75, chunk_vertices: int = CHUNK_VERTICES,
                        order: Optional[np.ndarray] = None):
     """The batch with every set's curves, streamed out set by set.
 
     Sets are admitted in set order while their slots (size x R) fit the
-    vertex arena; the pool advances every admitted particle by at most
-    pass_iterations per pass (pa_gpu_pool_run_vertices: each particle writes
-    its one-block reference slots into its set's part of the arena).  After
-    each pass, a set whose particles have all ended is finished on the device
+    vertex arena (by default up to 30% of the free device memory); the pool
+    advances every admitted particle by at most pass_iterations per pass (0:
+    to its end; pa_gpu_pool_run_vertices: each particle writes its one-block
+    reference slots into its set's part of the arena).  Bounded passes hand
+    sets out sooner but end each pass in a latency-bound tail (on the batched
+    workload: passes of 400 iterations 237 ms of pool time, to the end 127
+    ms).  After each pass, a set whose particles have all ended is finished on the device
     in chunks of at most chunk_vertices slots, as the reference chunks its
     curves (chunk = chunk_vertices // R seeds): counted prune, index, colour;
     each chunk goes to consumer(CurveChunk), then the set's arena range takes
@@ -273,8 +276,9 @@ def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField
     if not np.array_equal(sid, np.repeat(np.arange(len(sizes)), sizes)):
         raise ValueError("run_batch_vertices: the particles must be set by set (set_order)")
     need = sizes * Rs
-    if arena_slots is None:
-        arena_slots = int(max(need.max(), min(need.sum(), 1 << 31)))
+    if arena_slots is None:  # up to 30% of the free device memory
+        free = ctx.device_info()["free_bytes"]
+        arena_slots = int(max(need.max(), min(need.sum(), int(0.3 * free) // 16)))
     if need.max() > arena_slots:
         raise ValueError("a set needs %d slots, the arena holds %d" % (need.max(), arena_slots))
     if chunk_vertices < Rs.max():

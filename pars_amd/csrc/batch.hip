@@ -20,6 +20,7 @@ namespace pa_amd {
 namespace {
 
 constexpr int kPoolBlock = 256;
+constexpr int kStage = 4;  // vertices a lane stages in LDS before writing them
 constexpr int kMaxFields = 16;
 
 struct PoolConfig {
@@ -81,6 +82,11 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
   const int lane = threadIdx.x & 63;
   bool have = false, open = true;
   int64_t i = 0, vb = 0;
+  // Vertex output is staged per lane in LDS and written kStage slots at a time: a global store counts
+  // in the same in-order wait as the next gather, so one per step would lengthen every step.
+  __shared__ float4 stage[kStage][kPoolBlock];
+  int staged = 0;       // vertices waiting in LDS
+  int64_t sfirst = 0;   // slot of the first of them
   float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   int it = 0, R = 0, st = 0, left = 0;
   FieldDesc f = cfg.f[0];
@@ -116,7 +122,13 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
             left = budget;
             have = true;
             vb = b_idx;
-            if (V.slots && it == 1) V.slots[vb] = p;  // first load: the seed (initialize, :243-252)
+            staged = 0;
+            sfirst = vb + it;
+            if (V.slots && it == 1) {  // first load: the seed (initialize, :243-252)
+              stage[0][threadIdx.x] = p;
+              staged = 1;
+              sfirst = vb;
+            }
           }
         }
       }
@@ -132,7 +144,15 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
         if (advance_rt<S, MODE, INTERP>(cfg.k, f, p, next, rule) != 0) {
           fin = true;
         } else {
-          if (V.slots) V.slots[vb + it] = next;
+          if (V.slots) {
+            stage[staged][threadIdx.x] = next;
+            if (++staged == kStage) {
+#pragma unroll
+              for (int k = 0; k < kStage; ++k) V.slots[sfirst + k] = stage[k][threadIdx.x];
+              sfirst += kStage;
+              staged = 0;
+            }
+          }
           p = next;
           ++it;
           ++st;
@@ -143,10 +163,13 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
         pos[i] = p;
         it_a[i] = it;
         steps_a[i] = st;
-        if (fin && V.slots) {
-          // the termination vertex at the last slot reached (none for R < 2: the seed alone)
-          if (R >= 2) V.slots[vb + it] = make_float4(-2.0f, -2.0f, -2.0f, 0.0f);
-          V.count[i] = R >= 2 ? it + 1 : 1;
+        if (V.slots) {
+          // the termination vertex at the last slot reached (none for R < 2: the seed alone); what is
+          // staged goes out at the end of the particle's pass
+          if (fin && R >= 2) stage[staged++][threadIdx.x] = make_float4(-2.0f, -2.0f, -2.0f, 0.0f);
+          for (int k = 0; k < staged; ++k) V.slots[sfirst + k] = stage[k][threadIdx.x];
+          staged = 0;
+          if (fin) V.count[i] = R >= 2 ? it + 1 : 1;
         }
         if (fin) {
           status_a[i] = PA_POOL_DONE;
