@@ -215,6 +215,10 @@ pa_status   pa_gpu_field_load(pa_gpu_ctx* ctx, const char* path, const int64_t o
 /* Decoded FP32 node values back to the host, [d0][d1][d2][3]. */
 pa_status   pa_gpu_field_read(pa_gpu_ctx* ctx, const pa_gpu_field* field, float* h_xyz);
 pa_status   pa_gpu_field_bytes(const pa_gpu_field* field, size_t* bytes);
+/* vector_field's public members (vector_field.hpp:14-24): ghosted node
+ * counts, offset, size, spacing, and the storage type (any may be NULL). */
+pa_status   pa_gpu_field_info(const pa_gpu_field* field, int32_t dims[3], float offset[3], float size[3],
+                              float spacing[3], pa_storage* storage);
 pa_status   pa_gpu_field_destroy(pa_gpu_ctx* ctx, pa_gpu_field* field);
 
 /* vector_field::contains + interpolate (pa/source/math/vector_field.cpp:12-48)

@@ -297,6 +297,19 @@ pa_status pa_gpu_field_set_layout(pa_gpu_ctx* ctx, pa_gpu_field* field, pa_layou
   return PA_OK;
 }
 
+pa_status pa_gpu_field_info(const pa_gpu_field* field, int32_t dims[3], float offset[3], float size[3],
+                            float spacing[3], pa_storage* storage) {
+  if (!field) return PA_E_INVALID;
+  for (int i = 0; i < 3; ++i) {
+    if (dims) dims[i] = field->desc.dims[i];
+    if (offset) offset[i] = field->desc.offset[i];
+    if (size) size[i] = field->desc.size[i];
+    if (spacing) spacing[i] = field->desc.spacing[i];
+  }
+  if (storage) *storage = pa_storage(field->desc.storage);
+  return PA_OK;
+}
+
 pa_status pa_gpu_field_bytes(const pa_gpu_field* field, size_t* bytes) {
   if (!field || !bytes) return PA_E_INVALID;
   *bytes = field->bytes + field->cell_bytes;

@@ -113,10 +113,32 @@ class vector_field {
   // cache line per trilinear gather.
   void set_layout(pa_layout layout) { c_->check(pa_gpu_field_set_layout(c_->get(), f_, layout), "set_layout"); }
 
+  // data_io::load_vector_field (data_io.cpp:91-115): the ghosted hyperslab of
+  // the file's "vectors" dataset at a rank's node offset, spacing normalised
+  // by its largest component, offset and size as data_io.cpp:113-114 has
+  // them; read into pinned memory and stored on the device.
+  static std::unique_ptr<vector_field> load(context& c, const std::string& path, const std::array<int64_t, 3>& offset,
+                                            const std::array<int32_t, 3>& ghosted,
+                                            const std::array<int32_t, 3>& block,
+                                            pa_storage storage = PA_STORAGE_FP32) {
+    pa_gpu_field* f = nullptr;
+    c.check(pa_gpu_field_load(c.get(), path.c_str(), offset.data(), ghosted.data(), block.data(), storage, &f),
+            "field_load");
+    return std::unique_ptr<vector_field>(new vector_field(c, f));
+  }
+
   vector3 offset, size, spacing;
   std::array<int32_t, 3> dims;
 
  private:
+  vector_field(context& c, pa_gpu_field* f) : c_(&c), f_(f) {
+    float o[3], sz[3], h[3];
+    c.check(pa_gpu_field_info(f, dims.data(), o, sz, h, nullptr), "field_info");
+    offset = {o[0], o[1], o[2]};
+    size = {sz[0], sz[1], sz[2]};
+    spacing = {h[0], h[1], h[2]};
+  }
+
   context* c_;
   pa_gpu_field* f_ = nullptr;
 };
@@ -147,6 +169,7 @@ class particle_tracer {
   };
 
   explicit particle_tracer(context& c) : c_(&c) { c.check(pa_gpu_tracer_create(c.get(), &t_), "tracer_create"); }
+  context& owner() const { return *c_; }
   ~particle_tracer() {
     if (t_) pa_gpu_tracer_destroy(t_);
   }
@@ -169,6 +192,12 @@ class particle_tracer {
   }
   void set_interpolation(pa_interp i) { c_->check(pa_gpu_tracer_set_interpolation(t_, i), "set_interpolation"); }
   void set_engine(pa_engine e, int32_t waves = 0) { c_->check(pa_gpu_tracer_set_engine(t_, e, waves), "set_engine"); }
+  // Extension: the hybrid engine's launch constants and the XCD workgroup
+  // order (speed only; the results do not change).
+  void set_launch(int32_t quad_switch, int32_t pair_live, int32_t xcd_order = 1) {
+    c_->check(pa_gpu_tracer_set_launch(t_, quad_switch, pair_live), "set_launch");
+    c_->check(pa_gpu_tracer_set_xcd_remap(t_, xcd_order), "set_xcd_remap");
+  }
   void set_neighbor_face_masks(const std::array<int32_t, 6>& masks) {
     c_->check(pa_gpu_tracer_set_neighbor_face_masks(t_, masks.data()), "set_neighbor_face_masks");
   }
@@ -309,6 +338,20 @@ class particle_pool {
               "pool_run");
     return steps;
   }
+  // Extension: run() that also keeps each entry's curve (pa_gpu_pool_run_vertices): entry i writes its
+  // one-block round's slots from slots[base[i]] on and its vertex count to counts[i].
+  uint64_t run_vertices(particle_tracer& tracer, const std::vector<const vector_field*>& fields,
+                        const device_vector<int64_t>& base, device_vector<vector4>& slots,
+                        device_vector<int32_t>& counts, int32_t max_iterations = 0, int32_t waves_per_cu = 0) {
+    std::vector<const pa_gpu_field*> f;
+    for (auto* v : fields) f.push_back(v->get());
+    uint64_t steps = 0;
+    c_->check(pa_gpu_pool_run_vertices(tracer.get(), f.data(), int32_t(f.size()), &pool_, n_, max_iterations,
+                                       waves_per_cu, base.data(), reinterpret_cast<float*>(slots.data()),
+                                       slots.size(), counts.data(), &steps),
+              "pool_run_vertices");
+    return steps;
+  }
   std::vector<vector4> positions() const { return position_.to_host(n_); }
   std::vector<int32_t> steps() const { return steps_.to_host(n_); }
   std::vector<int32_t> status() const { return status_.to_host(n_); }
@@ -320,6 +363,151 @@ class particle_pool {
   device_vector<vector4> position_;
   device_vector<int32_t> iteration_, remaining_, steps_, status_, field_, request_, request_done_;
   pa_pool pool_{};
+};
+
+// Extension: rounds of a round loop (pars/source/pipeline.cpp:153-209) kept
+// in flight: each submitted round traces on the context's stream and
+// finishes (prune, index, colour) on its second stream while the next round
+// traces (pa_gpu_tracer_trace_union_round_pipelined).  Two sets of device
+// buffers rotate, so a round's result can be read until two more rounds are
+// submitted.
+class round_pipeline {
+ public:
+  round_pipeline(context& c, particle_tracer& tracer, size_t max_seeds, size_t region_slots,
+                 pa_color_mode mode = PA_COLOR_HSV_CONSTANT_S, scalar parameter = 0.75f)
+      : c_(&c), t_(&tracer), n_max_(max_seeds), region_(region_slots), mode_(mode), param_(parameter) {
+    for (auto& b : sets_) b.reset(new buffers(c, max_seeds, region_slots));
+  }
+  // Queue one round over `seeds`; returns its ticket.
+  size_t submit(const std::vector<particle>& seeds) {
+    if (seeds.size() > n_max_) throw error(PA_E_CAPACITY, "round_pipeline: more seeds than max_seeds");
+    buffers& b = *sets_[next_ % 2];
+    b.n = seeds.size();
+    if (b.n) c_->check(pa_gpu_memcpy_h2d(c_->get(), b.seeds.data(), seeds.data(), b.n * sizeof(particle)), "h2d");
+    c_->check(pa_gpu_tracer_trace_union_round_pipelined(
+                  t_->get(), b.seeds.data(), b.n, region_, reinterpret_cast<float*>(b.slots.data()),
+                  b.counts.data(), b.cont.data(), mode_, param_, reinterpret_cast<float*>(b.vertices.data()), b.cap,
+                  b.indices.data(), b.cap, reinterpret_cast<float*>(b.colors.data()),
+                  reinterpret_cast<float*>(b.ends.data()), nullptr, b.result.data()),
+              "trace_union_round_pipelined");
+    return next_++;
+  }
+  // Wait for a round (one of the last two submitted) and copy its results.
+  particle_tracer::union_round result(size_t ticket) {
+    if (ticket >= next_ || ticket + 2 < next_) throw error(PA_E_STATE, "round_pipeline: round no longer held");
+    buffers& b = *sets_[ticket % 2];
+    c_->check(pa_gpu_synchronize(c_->get()), "synchronize");
+    const pa_round_result r = b.result.to_host(1)[0];
+    c_->check(pa_gpu_round_result_check(c_->get(), &r, b.cap, b.cap), "round");
+    particle_tracer::union_round out;
+    if (b.n == 0) return out;
+    out.total_steps = r.total_steps;
+    out.curves.vertices = b.vertices.to_host(size_t(r.n_vertices));
+    out.curves.indices = b.indices.to_host(size_t(r.n_indices));
+    out.curves.colors = b.colors.to_host(size_t(r.n_vertices));
+    out.continuation = b.cont.to_host(r.n_continuation);
+    out.counts = b.counts.to_host(b.n);
+    out.endpoints = b.ends.to_host(b.n);
+    return out;
+  }
+
+ private:
+  struct buffers {
+    buffers(context& c, size_t n, size_t region)
+        : cap(n * region), seeds(c, n), slots(c, (n + 7) / 8 * 8 * region), counts(c, n), cont(c, n), ends(c, n),
+          vertices(c, n * region), indices(c, n * region), colors(c, n * region), result(c, 1) {}
+    size_t n = 0, cap;
+    device_vector<particle> seeds;
+    device_vector<vector4> slots;
+    device_vector<int32_t> counts;
+    device_vector<particle> cont;
+    device_vector<vector4> ends;
+    device_vector<vector4> vertices;
+    device_vector<int32_t> indices;
+    device_vector<vector4> colors;
+    device_vector<pa_round_result> result;
+  };
+  context* c_;
+  particle_tracer* t_;
+  size_t n_max_, region_;
+  pa_color_mode mode_;
+  scalar param_;
+  size_t next_ = 0;
+  std::unique_ptr<buffers> sets_[2];
+};
+
+// Extension: the device-resident serving loop (pa_gpu_serve_*): requests are
+// published into a pinned ring as they arrive and run on a persistent kernel;
+// each particle ends where a one-block trace on its field ends (pars serves
+// one request at a time, pars/source/service.cpp:20-69).
+class resident_server {
+ public:
+  resident_server(particle_tracer& tracer, const std::vector<const vector_field*>& fields, size_t capacity,
+                  int32_t request_slots = 1024, int32_t waves_per_cu = 0)
+      : c_(&tracer.owner()) {
+    std::vector<const pa_gpu_field*> f;
+    for (auto* v : fields) f.push_back(v->get());
+    c_->check(pa_gpu_serve_create(tracer.get(), f.data(), int32_t(f.size()), capacity, request_slots, waves_per_cu, 0,
+                                  &s_),
+              "serve_create");
+  }
+  ~resident_server() {
+    if (s_) pa_gpu_serve_destroy(s_);
+  }
+  resident_server(const resident_server&) = delete;
+  resident_server& operator=(const resident_server&) = delete;
+  // Publish a request; false when the ring or the slots are full for now.
+  bool submit(const std::vector<particle>& particles, int32_t* request) {
+    const pa_status st = pa_gpu_serve_submit(s_, particles.data(), particles.size(), request);
+    if (st == PA_E_CAPACITY && particles.size() > 0) return false;
+    c_->check(st, "serve_submit");
+    return true;
+  }
+  bool done(int32_t request) {
+    int32_t d = 0;
+    c_->check(pa_gpu_serve_poll(s_, request, &d), "serve_poll");
+    return d != 0;
+  }
+  // A done request's endpoints and step counts; its slot and ring space are free again.
+  void collect(int32_t request, size_t n, std::vector<vector4>* endpoints, std::vector<int32_t>* steps) {
+    if (endpoints) endpoints->resize(n);
+    if (steps) steps->resize(n);
+    uint64_t sum = 0;
+    c_->check(pa_gpu_serve_collect(s_, request, endpoints ? &(*endpoints)[0].x : nullptr,
+                                   steps ? steps->data() : nullptr, &sum),
+              "serve_collect");
+  }
+  // Drain and stop the kernel (before other GPU work of the process); total steps served.
+  uint64_t stop() {
+    uint64_t steps = 0, launches = 0;
+    c_->check(pa_gpu_serve_stop(s_, &steps, &launches), "serve_stop");
+    return steps;
+  }
+
+ private:
+  context* c_;
+  pa_gpu_server* s_ = nullptr;
+};
+
+// Extension: particles as the records boost::mpi sends (particle.hpp:16-28,
+// 40 bytes each), converted on the device.
+struct wire {
+  static std::vector<uint8_t> to_wire(context& c, const std::vector<particle>& particles) {
+    std::vector<uint8_t> out(particles.size() * 40);
+    if (particles.empty()) return out;
+    device_vector<particle> d(c, particles);
+    device_vector<uint8_t> w(c, out.size());
+    c.check(pa_gpu_particles_to_wire(c.get(), d.data(), particles.size(), w.data()), "to_wire");
+    return w.to_host(out.size());
+  }
+  static std::vector<particle> from_wire(context& c, const std::vector<uint8_t>& records) {
+    const size_t n = records.size() / 40;
+    if (n == 0) return {};
+    device_vector<uint8_t> w(c, records);
+    device_vector<particle> d(c, n);
+    c.check(pa_gpu_particles_from_wire(c.get(), w.data(), n, d.data()), "from_wire");
+    return d.to_host(n);
+  }
 };
 
 // pa::index_generator (index_generator.hpp:10-14).
