@@ -173,9 +173,13 @@ pa_status make_field(pa_gpu_ctx* ctx, const int32_t dims[3], const float offset[
     d.dims[i] = dims[i];
     d.offset[i] = offset[i];
     d.spacing[i] = spacing[i];
-    d.size[i] = size[i];
+    f->size[i] = size[i];
+    d.cell_hi[i] = static_cast<float>(dims[i] - 2);
     d.upper[i] = offset[i] + size[i];  // particle_tracer.cpp:263, one FP32 add
     d.inv_spacing[i] = 1.0f / spacing[i];
+    // 1/h as an unevaluated sum hi + lo: the FP64 reciprocal is within 2^-53
+    // of 1/h, so lo is RN(1/h - hi) up to a part in 2^29 of itself
+    d.inv_lo[i] = static_cast<float>(1.0 / double(spacing[i]) - double(d.inv_spacing[i]));
   }
   d.fast_div = 1;
   for (int i = 0; i < 3; ++i)
@@ -303,7 +307,7 @@ pa_status pa_gpu_field_info(const pa_gpu_field* field, int32_t dims[3], float of
   for (int i = 0; i < 3; ++i) {
     if (dims) dims[i] = field->desc.dims[i];
     if (offset) offset[i] = field->desc.offset[i];
-    if (size) size[i] = field->desc.size[i];
+    if (size) size[i] = field->size[i];
     if (spacing) spacing[i] = field->desc.spacing[i];
   }
   if (storage) *storage = pa_storage(field->desc.storage);

@@ -47,26 +47,59 @@ __device__ __forceinline__ float dec16(uint32_t h, int storage) {
   return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h & 0xffffu)));
 }
 
-// Correctly rounded a / h from y = RN(1/h) without the division sequence.
-// q0 = RN(a y) is within 1.5 ulp of a/h; one FMA correction brings it within
-// one ulp, and a second is then RN(a/h) by Markstein's theorem (the residual
-// a - q h of a quotient within one ulp is exact, and RN(q + r y) with y the
+// Correctly rounded a / h from 1/h as an unevaluated sum y + yl (y =
+// RN(1/h), yl ~ RN(1/h - y)) without the division sequence.  q0 = RN(a y +
+// RN(a yl)) is within a hair of half an ulp of a/h, so within one ulp, and
+// one FMA correction is then RN(a/h) by Markstein's theorem (the residual a
+// - q0 h of a quotient within one ulp is exact, and RN(q0 + r y) with y the
 // correctly rounded reciprocal is the correctly rounded quotient; P.
 // Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
 // Floating-Point Arithmetic, 2nd ed., sec. 4.7).  It needs no intermediate
 // underflow or overflow: the callers take it only for a in {0} or
-// [2^-40, 2^40] with h in [2^-20, 2^20].  The sign of a zero quotient is
-// restored explicitly (h > 0).
-__device__ __forceinline__ float div_corrected(float a, float h, float y) {
-  const float q0 = a * y;
-  const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, h, a), y, q0);
-  const float q2 = __builtin_fmaf(__builtin_fmaf(-q1, h, a), y, q1);
-  return __builtin_copysignf(q2, a);
+// [2^-40, 2^40] with h in [2^-20, 2^20].  A zero a may give a zero of either
+// sign: div_corrected restores the sign of a; div_unsigned leaves it to
+// callers that give the result its sign themselves or only floor and
+// compare it (where -0 and +0 act alike).
+__device__ __forceinline__ float div_unsigned(float a, float h, float y, float yl) {
+  const float q0 = __builtin_fmaf(a, y, a * yl);
+  return __builtin_fmaf(__builtin_fmaf(-q0, h, a), y, q0);
+}
+__device__ __forceinline__ float div_corrected(float a, float h, float y, float yl) {
+  return __builtin_copysignf(div_unsigned(a, h, y, yl), a);
 }
 
 __device__ __forceinline__ bool div_range(float a) {
   const float x = fabsf(a);
   return (int(x == 0.0f) | (int(x >= 0x1p-40f) & int(x <= 0x1p40f))) != 0;
+}
+
+// fmodf(d, h) of a located point from its cell subscript m = floor(q),
+// q = RN(d / h), for d >= 0 (or -0) with d / h not underflowing (the
+// callers' range check): q rounds the true quotient t either to within
+// [m, m + 1), where d - m h is the remainder, or up to the integer m from
+// just below, where d - m h lies in (-h, 0) and the remainder is
+// d - m h + h.  Both are exact in FP32 (a remainder is always
+// representable, and d - m h then is a small multiple of ulp(d)), so the
+// FMA and the add round nothing.  A zero remainder may come out as -0 or
+// +0: the caller gives the weight the sign of d (fmodf's sign rule).
+__device__ __forceinline__ float rem_located(float d, float h, float m) {
+  const float r0 = __builtin_fmaf(-m, h, d);
+  return r0 < 0.0f ? r0 + h : r0;
+}
+
+// A remainder r in [0, h) that div_corrected may divide is 0 or >= 2^-40,
+// i.e. rem_key(r) >= kRemKeyMin (unsigned): a zero's key wraps to the top,
+// one of (0, 2^-40) falls below.  The keys of three remainders take one
+// unsigned minimum.
+__device__ __forceinline__ uint32_t rem_key(float r) { return __float_as_uint(r) - 1u; }
+constexpr uint32_t kRemKeyMin = 0x2B800000u - 1u;  // bits of 2^-40, minus 1
+__device__ __forceinline__ bool rem_range(float r) { return rem_key(r) >= kRemKeyMin; }
+
+// fmod(d, h) / h of the reference's weight (vector_field.cpp:28-30) from
+// the remainder r = rem_located(d, h, floor(q)): correctly rounded, with
+// the sign of d, for d and r within div_corrected's ranges.
+__device__ __forceinline__ float weight_located(float d, float h, float y, float yl, float r) {
+  return __builtin_copysignf(div_unsigned(r, h, y, yl), d);
 }
 
 // Exact fmodf(d, h) for h > 0 and a located point (0 <= q = d/h rounded <
@@ -84,61 +117,80 @@ __device__ __forceinline__ float exact_fmod(float d, float h, float q) {
 // vector_field::contains (vector_field.cpp:12-21) sharing the quotients
 // with interpolate: q_i = (p_i - o_i) / h_i; inside iff 0 <= floor(q_i) <=
 // dims_i - 2 (a negative or NaN subscript fails through the reference's
-// size_t cast).
-__device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, float pz, float d[3], float q[3]) {
+// size_t cast).  A zero quotient may carry either sign: every use floors
+// it, compares it or subtracts its floor, where -0 and +0 act alike.
+// fast: d and the spacing were within div_corrected's ranges (cell_weights
+// takes it).
+__device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, float pz, float d[3], float q[3],
+                                       bool& fast) {
   d[0] = px - f.offset[0];
   d[1] = py - f.offset[1];
   d[2] = pz - f.offset[2];
   // bitwise on purpose: no short-circuit branches
-  const bool fast = (int(f.fast_div != 0) & int(div_range(d[0])) & int(div_range(d[1])) & int(div_range(d[2]))) != 0;
+  fast = (int(f.fast_div != 0) & int(div_range(d[0])) & int(div_range(d[1])) & int(div_range(d[2]))) != 0;
   if (__builtin_expect(fast, 1)) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) q[i] = div_corrected(d[i], f.spacing[i], f.inv_spacing[i]);
+    for (int i = 0; i < 3; ++i) q[i] = div_unsigned(d[i], f.spacing[i], f.inv_spacing[i], f.inv_lo[i]);
   } else {
 #pragma unroll
     for (int i = 0; i < 3; ++i) q[i] = d[i] / f.spacing[i];
   }
-  bool inside = true;
+  int inside = 1;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float s = floorf(q[i]);
-    inside = inside && (s >= 0.0f) && (s <= static_cast<float>(f.dims[i] - 2));
+    inside &= int(s >= 0.0f) & int(s <= f.cell_hi[i]);
   }
-  return inside;
+  return inside != 0;
+}
+__device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, float pz, float d[3], float q[3]) {
+  bool fast;
+  return locate(f, px, py, pz, d, q, fast);
 }
 
 __device__ __forceinline__ float lerp1(float a, float b, float w, float u) { return u * a + w * b; }
 
 // Cell index and weights of vector_field::interpolate (vector_field.cpp:
-// 27-31): m = floor(q); w = fmod(d, h) / h (reference) or q - m
-// (consistent); u = 1 - w.
+// 27-31) at a located point: m = floor(q); w = fmod(d, h) / h (reference)
+// or q - m (consistent); u = 1 - w.  The reference weight takes the
+// remainder from m (rem_located) and one corrected quotient while d and the
+// remainder are within div_corrected's ranges, else fmodf and division.
+// dfast: locate's flag for d; unused: the caller will not use these weights
+// (a lane that does not step), so they never take the slow path.
 template <int INTERP>
-__device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3], const float q[3], int m[3],
-                                             float w[3], float u[3]) {
-  float r[3];
-  bool fast = true;
+__device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3], const float q[3], bool dfast,
+                                             int m[3], float w[3], float u[3], bool unused = false) {
+  float fl[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const float fl = floorf(q[i]);
-    m[i] = static_cast<int>(fl);
-    if constexpr (INTERP == PA_INTERP_CONSISTENT) {
-      w[i] = q[i] - fl;
-    } else {
-      r[i] = exact_fmod(d[i], f.spacing[i], q[i]);
-      fast = (int(fast) & int(div_range(r[i]))) != 0;
-    }
+    fl[i] = floorf(q[i]);
+    m[i] = static_cast<int>(fl[i]);
   }
-  if constexpr (INTERP != PA_INTERP_CONSISTENT) {
-    if (__builtin_expect((int(fast) & int(f.fast_div != 0)) != 0, 1)) {
+  if constexpr (INTERP == PA_INTERP_CONSISTENT) {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) w[i] = div_corrected(r[i], f.spacing[i], f.inv_spacing[i]);
+    for (int i = 0; i < 3; ++i) w[i] = q[i] - fl[i];
+  } else {
+    float r[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = rem_located(d[i], f.spacing[i], fl[i]);
+    const uint32_t key = min(min(rem_key(r[0]), rem_key(r[1])), rem_key(r[2]));
+    if (__builtin_expect((int(unused) | (int(dfast) & int(key >= kRemKeyMin))) != 0, 1)) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) w[i] = weight_located(d[i], f.spacing[i], f.inv_spacing[i], f.inv_lo[i], r[i]);
     } else {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) w[i] = r[i] / f.spacing[i];
+      for (int i = 0; i < 3; ++i) w[i] = exact_fmod(d[i], f.spacing[i], q[i]) / f.spacing[i];
     }
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) u[i] = 1.0f - w[i];
+}
+template <int INTERP>
+__device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3], const float q[3], int m[3],
+                                             float w[3], float u[3]) {
+  const bool dfast =
+      (int(f.fast_div != 0) & int(div_range(d[0])) & int(div_range(d[1])) & int(div_range(d[2]))) != 0;
+  cell_weights<INTERP>(f, d, q, dfast, m, w, u);
 }
 
 // The lerp tree of vector_field.cpp:41-47 over the 8 corners
@@ -199,12 +251,7 @@ __device__ __forceinline__ void decode_record(const uint4* r, float k[24]) {
 // The lerp tree of vector_field.cpp:41-47 (z, then y, then x) over
 // component-planar corners k[comp * 8 + corner], corner order c000 c001
 // c010 c011 c100 c101 c110 c111.
-template <int INTERP>
-__device__ __forceinline__ void lerp_planar(const FieldDesc& f, const float d[3], const float q[3], const float k[24],
-                                            float v[3]) {
-  int m[3];
-  float w[3], u[3];
-  cell_weights<INTERP>(f, d, q, m, w, u);
+__device__ __forceinline__ void lerp_weighted(const float k[24], const float w[3], const float u[3], float v[3]) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float* c = k + 8 * i;
@@ -213,6 +260,14 @@ __device__ __forceinline__ void lerp_planar(const FieldDesc& f, const float d[3]
     const float c0 = lerp1(c00, c01, w[1], u[1]), c1 = lerp1(c10, c11, w[1], u[1]);
     v[i] = lerp1(c0, c1, w[0], u[0]);
   }
+}
+template <int INTERP>
+__device__ __forceinline__ void lerp_planar(const FieldDesc& f, const float d[3], const float q[3], const float k[24],
+                                            float v[3]) {
+  int m[3];
+  float w[3], u[3];
+  cell_weights<INTERP>(f, d, q, m, w, u);
+  lerp_weighted(k, w, u, v);
 }
 
 // The loads of one cell's 8 corners as issued (gather_issue), decoded later

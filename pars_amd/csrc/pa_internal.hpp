@@ -28,7 +28,7 @@ struct FieldDesc {
   float offset[3];
   float spacing[3];
   float upper[3];
-  float size[3];
+  float cell_hi[3];   // dims - 2 as FP32: the largest cell subscript
   int64_t stride0;    // dims[1] * dims[2]
   int64_t stride1;    // dims[2]
   const void* cells;  // cell records (PA_LAYOUT_CELLS) or null
@@ -37,6 +37,7 @@ struct FieldDesc {
   int32_t layout;     // pa_layout
   float inv_spacing[3];  // RN(1 / spacing), for the corrected quotient
   int32_t fast_div;      // spacing within [2^-20, 2^20] on every axis
+  float inv_lo[3];       // RN(1 / spacing - inv_spacing): the reciprocal's low part
 };
 
 // Vec4 index of slot s of seed p in 8-seed grouped slots (PA_SLOTS_GROUP8).
@@ -67,6 +68,7 @@ struct TraceConfig {
 
 struct pa_gpu_field {
   pa_amd::FieldDesc desc;
+  float size[3] = {0.0f, 0.0f, 0.0f};  // the block's extent as the caller gave it
   void* d_nodes = nullptr;
   size_t bytes = 0;
   void* d_cells = nullptr;

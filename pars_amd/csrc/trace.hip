@@ -1237,7 +1237,8 @@ pa_status pa_gpu_flow_map_generate(pa_gpu_tracer* t, const int32_t block_size[3]
     spacing[i] = f.spacing[i] / resolution_scale;                     // :253-256
   }
   size_t ns = 0;
-  if (pa_gpu_seed_count(f.size, spacing, sd, &ns) != PA_OK) return fail(ctx, PA_E_INVALID, "bad flow-map lattice");
+  if (pa_gpu_seed_count(t->local->size, spacing, sd, &ns) != PA_OK)
+    return fail(ctx, PA_E_INVALID, "bad flow-map lattice");
   const int64_t nmap = int64_t(dims_out[0]) * dims_out[1] * dims_out[2];
   if (nmap <= 0) return fail(ctx, PA_E_INVALID, "empty flow map");
   if (!d_flow_map) return PA_E_INVALID;

@@ -60,7 +60,7 @@ __device__ __forceinline__ bool quad_all(bool b) {
 // Per-component view of one block (lane c of the quad holds component c;
 // the w lane holds neutral values that never fail a test).
 struct QEntry {
-  float off, h, inv, hi, up;
+  float off, h, inv, inv_lo, hi, up;
   uint32_t stride;  // cells: cstride0, cstride1, 1, 0; nodes: stride0, stride1, 1, 0
   uint32_t st0, st1;
 };
@@ -74,7 +74,7 @@ struct QBlock {
 
 struct QLane {
   const void* data;
-  float off, h, inv, hi, up;
+  float off, h, inv, inv_lo, hi, up;
   uint32_t stride, st0, st1;
   int32_t fast, mask;
 };
@@ -86,6 +86,7 @@ __device__ __forceinline__ void load_lane(const QEntry (*tab)[4], const QBlock* 
   L.off = e.off;
   L.h = e.h;
   L.inv = e.inv;
+  L.inv_lo = e.inv_lo;
   L.hi = e.hi;
   L.up = e.up;
   L.stride = e.stride;
@@ -126,17 +127,22 @@ __device__ __forceinline__ void decode_quad(const float4& A, const float4& B, fl
 
 // Weights (vector_field.cpp:27-31) and the lerp tree (:41-47) over the
 // quad: lane c weighs along its axis; the result is component c of the
-// velocity (0 on the w lane).
+// velocity (0 on the w lane).  unused: the lane's weight is not used (the
+// w lane, or a quad that does not step), so it never takes the slow path.
 template <int INTERP>
-__device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, float q, float s, const float k[8]) {
+__device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, float q, float s, bool dfast,
+                                          const float k[8], bool unused) {
   float w;
   if constexpr (INTERP == PA_INTERP_CONSISTENT) {
     w = q - s;
   } else {
-    const float r = exact_fmod(d, L.h, q);
-    // per lane: both branches give the correctly rounded quotient
-    if (__builtin_expect((int(wl | div_range(r)) & int(L.fast != 0)) != 0, 1)) w = div_corrected(r, L.h, L.inv);
-    else w = r / L.h;
+    // as cell_weights: the remainder from s = floor(q) while d and it are
+    // within div_corrected's ranges, else fmodf and division
+    const float r = rem_located(d, L.h, s);
+    if (__builtin_expect((int(unused) | (int(dfast) & int(rem_range(r)))) != 0, 1))
+      w = weight_located(d, L.h, L.inv, L.inv_lo, r);
+    else
+      w = exact_fmod(d, L.h, q) / L.h;
   }
   const float w0 = qmovf<kBcast0>(w), w1 = qmovf<kBcast1>(w), w2 = qmovf<kBcast2>(w);
   const float u0 = 1.0f - w0, u1 = 1.0f - w1, u2 = 1.0f - w2;
@@ -149,11 +155,14 @@ __device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, flo
 // The quotient of vector_field::contains (vector_field.cpp:12-21) along
 // the lane's axis: d = x - offset, q = d / h, s = floor(q); quad-uniform
 // result, false = outside the block.
-__device__ __forceinline__ bool quad_locate(const QLane& L, bool wl, float x, float& d, float& q, float& s) {
+// fast: d and the spacing were within div_corrected's ranges (or the w lane).
+__device__ __forceinline__ bool quad_locate(const QLane& L, bool wl, float x, float& d, float& q, float& s,
+                                            bool& fast) {
   d = x - L.off;
   // per lane: both branches give the correctly rounded quotient
-  const bool fast = (int(wl | div_range(d)) & int(L.fast != 0)) != 0;
-  if (__builtin_expect(fast, 1)) q = div_corrected(d, L.h, L.inv);
+  fast = (int(wl | div_range(d)) & int(L.fast != 0)) != 0;
+  // (a zero quotient may carry either sign, as in locate)
+  if (__builtin_expect(fast, 1)) q = div_unsigned(d, L.h, L.inv, L.inv_lo);
   else q = d / L.h;
   s = floorf(q);
   return quad_all(wl | ((s >= 0.0f) & (s <= L.hi)));
@@ -166,7 +175,8 @@ template <int S, int INTERP>
 __device__ __forceinline__ bool quad_eval(const QLane& L, int c, float x, float& v) {
   const bool wl = c == 3;
   float d, q, s;
-  if (!quad_locate(L, wl, x, d, q, s)) return false;
+  bool dfast;
+  if (!quad_locate(L, wl, x, d, q, s, dfast)) return false;
   const uint32_t m = wl ? 0u : static_cast<uint32_t>(static_cast<int>(s));
   float k[8];  // component c of c000 c001 c010 c011 c100 c101 c110 c111
   if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
@@ -191,16 +201,17 @@ __device__ __forceinline__ bool quad_eval(const QLane& L, int c, float x, float&
       k[6] = dec16(gload<uint16_t>(D, uint64_t(n6) * 4 + c), st); k[7] = dec16(gload<uint16_t>(D, uint64_t(n6 + 1) * 4 + c), st);
     }
   }
-  v = quad_lerp<INTERP>(L, wl, d, q, s, k);
+  v = quad_lerp<INTERP>(L, wl, d, q, s, dfast, k, wl);
   return true;
 }
 
 // A quad's record for this iteration, gathered before the quad's state
 // machine runs (cell records): in = the quad steps and its point is inside
-// the block; d, q, s as quad_locate; A, B as load_record.
+// the block; d, q, s, fast as quad_locate; A, B as load_record.
 struct QGather {
   bool in;
   float d, q, s;
+  bool fast;
   float4 A, B;
 };
 
@@ -248,7 +259,7 @@ __device__ __forceinline__ int quad_advance(const TraceConfig& k, const QLane& L
     if (!G->in) return 1;
     float kc[8];
     decode_quad<S>(G->A, G->B, kc);
-    v = quad_lerp<INTERP>(L, c == 3, G->d, G->q, G->s, kc);
+    v = quad_lerp<INTERP>(L, c == 3, G->d, G->q, G->s, G->fast, kc, c == 3);
   } else if (!quad_eval<S, INTERP>(L, c, x, v)) {
     return 1;
   }
@@ -279,12 +290,14 @@ __device__ __forceinline__ void fill_tables(const TraceConfig& cfg, QEntry (*tab
       e.off = F.offset[c];
       e.h = F.spacing[c];
       e.inv = F.inv_spacing[c];
-      e.hi = static_cast<float>(F.dims[c] - 2);
+      e.inv_lo = F.inv_lo[c];
+      e.hi = F.cell_hi[c];
       e.up = F.upper[c];
     } else {
       e.off = 0.0f;
       e.h = 1.0f;
       e.inv = 1.0f;
+      e.inv_lo = 0.0f;
       e.hi = 3.0e38f;
       e.up = __builtin_inff();
     }
@@ -395,6 +408,64 @@ __device__ __forceinline__ float quad_iterate(const TraceConfig& cfg, const QEnt
   return val;
 }
 
+// quad_iterate for cell records and the frozen-derivative steppers as one
+// path of selects for every quad (a quad in the tail runs alone on its
+// SIMD, where each branch of the state machine is a stall on the chain);
+// only the hand-over of a quad that left its block stays a branch.  The
+// slots are those of quad_iterate, bit for bit.  G: this iteration's
+// gather (G.in: the quad steps and its point is inside the block).
+template <int S, int MODE, int INTERP>
+__device__ __forceinline__ float quad_iterate_sel(const TraceConfig& cfg, const QEntry (*tab)[4], const QBlock* blk,
+                                                  QState& st, QLane& L, float& last, int c, int rule, int64_t region,
+                                                  pa_particle* cont, unsigned* n_cont, const QGather& G) {
+  static_assert(MODE != STEP_CLASSIC, "classic RK4 evaluates the field at its stage points: quad_iterate");
+  const float term = c == 3 ? 0.0f : -2.0f;  // termination vertex (-2, -2, -2, 0)
+  const bool active = st.state != Q_IDLE;
+  const bool start = st.state == Q_START;
+  float kc[8];
+  decode_quad<S>(G.A, G.B, kc);
+  const float v = quad_lerp<INTERP>(L, c == 3, G.d, G.q, G.s, G.fast, kc, (c == 3) | !G.in);
+  const bool stop = quad_stops(v, c, rule, cfg.stop_eps2);
+  const float nx = MODE == STEP_EULER ? last + cfg.dt * v : frozen_rk4(cfg, last, v);
+  const bool stepped = G.in & !stop;
+  const bool left = active & !start & (st.it != st.R - 1) & !G.in;  // contains failed (quad-uniform)
+  const float val = start ? last : stepped ? nx : term;
+  int nstate = start ? (st.R < 2 ? Q_DONE : Q_STEP) : stepped ? Q_STEP : Q_DONE;
+  st.it = start ? 1 : st.it + int(stepped);
+  st.nsteps += int(stepped);
+  last = stepped ? nx : last;
+  st.slot += int(active);
+  if (left) {
+    const int face = quad_face(L, c, last);
+    int nb = -1;
+    bool go;
+    if (st.b < 0) {
+      go = face >= 0 && blk[face + 1].has != 0;
+      nb = face;
+    } else {
+      go = face == (st.b ^ 1);
+    }
+    if (go) {
+      const int Rn = st.R - st.it;
+      if (int64_t(st.slot) + Rn > region) {
+        unsigned idx = 0;
+        if (c == 0) idx = atomicAdd(n_cont, 1u);
+        idx = static_cast<unsigned>(qmov<kBcast0>(static_cast<int>(idx)));
+        const float lx = qmovf<kBcast0>(last), ly = qmovf<kBcast1>(last), lz = qmovf<kBcast2>(last),
+                    lw = qmovf<0xFF>(last);
+        if (c == 0) write_continuation(cont + idx, make_float4(lx, ly, lz, lw), Rn, nb, st.orank, st.pid);
+      } else {
+        st.b = nb;
+        st.R = Rn;
+        load_lane(tab, blk, st.b, c, L);
+        nstate = Q_START;
+      }
+    }
+  }
+  st.state = active ? nstate : Q_IDLE;
+  return val;
+}
+
 template <int S, int MODE, int INTERP>
 __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg, const pa_particle* __restrict__ P,
                                                            int64_t n, float* __restrict__ slots, int64_t region,
@@ -482,57 +553,80 @@ __global__ __launch_bounds__(kQuadBlock) void k_quad_union(const TraceConfig cfg
   if (lane == 0 && wave_steps) atomicAdd(steps_out, wave_steps);
 }
 
-// FP32 cell records, lane mode.  The six chunks a lane loads are three
-// from base a and three from base b.  Alone, a lane loads its own record (a
-// = record, b = record + 48 B).  While a wave is full (pair), the two lanes
-// of a pair (l, l ^ 1) share their loads: the even lane loads chunks 0-2 of
-// both records, the odd lane chunks 3-5, so each load instruction touches
-// one record line per pair instead of one per lane: the L1 looks up half as
-// many tags, which is what limits a full wave's gathers.  A DPP swap then
-// hands each lane the half of its own record that its partner loaded.  Both
-// forms use the same six loads, so their registers need no merge of paths.
+// FP32 cell records, lane mode.  A lane loads three chunks of record e
+// (X) and three of record o (Y).  Alone, both are the lane's own record:
+// X its chunks 0-2, Y its chunks 3-5.  While a wave is full (pair), the two
+// lanes of a pair (l, l ^ 1) share their loads: e is the even lane's
+// record and o the odd lane's, the even lane loads chunks 0-2 of both and
+// the odd lane chunks 3-5 of both, so each load instruction touches one
+// record line per pair instead of one per lane (the L1 looks up half as
+// many tags, which is what limits a full wave's gathers).  Each chunk holds
+// one component of the four corners of one x-half of the cell (x c000..c011,
+// x c100..c111, y .., y .., z .., z ..), which the lerp tree reduces on its
+// own (z, then y) to that half's value; only the x step combines chunks
+// 2k and 2k + 1.  So every lane reduces the chunks it loaded with the
+// weights of the record they belong to (the partner's weights by a DPP
+// swap), and the partner's three half values go back by one more swap:
+// three values cross the pair instead of twelve record words, and the
+// arithmetic is the lerp tree's own, operation for operation.
 struct PairGather {
-  uint4 A[3], B[3];
+  uint4 X[3], Y[3];
 };
+
+__device__ __forceinline__ float swapf(float v) { return qmovf<kSwap1>(v); }
 
 __device__ __forceinline__ void pair_issue(uint64_t r, bool pair, bool even, PairGather& P) {
   const uint32_t plo = static_cast<uint32_t>(qmov<kSwap1>(static_cast<int>(uint32_t(r))));
   const uint32_t phi = static_cast<uint32_t>(qmov<kSwap1>(static_cast<int>(r >> 32)));
   const uint64_t p = uint64_t(phi) << 32 | plo;
   const uint64_t a = pair && !even ? p + 48 : r;
-  const uint64_t b = !pair ? r + 48 : even ? p : r + 48;
+  const uint64_t b = pair && even ? p : r + 48;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    P.A[k] = gload<uint4>(reinterpret_cast<const uint4*>(a), k);
-    P.B[k] = gload<uint4>(reinterpret_cast<const uint4*>(b), k);
+    P.X[k] = gload<uint4>(reinterpret_cast<const uint4*>(a), k);
+    P.Y[k] = gload<uint4>(reinterpret_cast<const uint4*>(b), k);
   }
 }
 
-__device__ __forceinline__ uint32_t swap_pair(uint32_t v) {
-  return static_cast<uint32_t>(qmov<kSwap1>(static_cast<int>(v)));
+// One chunk's x-half value: the z lerps of its two corner pairs, then the
+// y lerp (vector_field.cpp:41-46 for one component and one x-half).
+__device__ __forceinline__ float chunk_half(const uint4 c, float w1, float u1, float w2, float u2) {
+  const float z0 = lerp1(__uint_as_float(c.x), __uint_as_float(c.y), w2, u2);
+  const float z1 = lerp1(__uint_as_float(c.z), __uint_as_float(c.w), w2, u2);
+  return lerp1(z0, z1, w1, u1);
 }
 
-__device__ __forceinline__ uint4 sel4(bool c, uint4 x, uint4 y) {  // value select (no lvalue select)
-  return make_uint4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
-}
-
-// The lane's own record (chunks 0-5).
-template <int S>
-__device__ __forceinline__ void pair_finish(const PairGather& P, bool pair, bool even, Gather<S>& g) {
+// The lane's velocity from a PairGather (all lanes take part: a lane that
+// does not step still reduces its partner's chunks).  w, u: the lane's own
+// weights (cell_weights).
+template <int INTERP>
+__device__ __forceinline__ void pair_lerp(const PairGather& P, bool pair, bool even, const float w[3], const float u[3],
+                                          float v[3]) {
+  // the partner's y and z weights (u = 1 - w, the same single rounding)
+  const float pw1 = swapf(w[1]), pw2 = swapf(w[2]);
+  const float pu1 = 1.0f - pw1, pu2 = 1.0f - pw2;
+  const bool xe = pair && !even;  // X belongs to the partner
+  const bool ye = pair && even;   // Y belongs to the partner
+  const float xw1 = xe ? pw1 : w[1], xu1 = xe ? pu1 : u[1], xw2 = xe ? pw2 : w[2], xu2 = xe ? pu2 : u[2];
+  const float yw1 = ye ? pw1 : w[1], yu1 = ye ? pu1 : u[1], yw2 = ye ? pw2 : w[2], yu2 = ye ? pu2 : u[2];
+  float hx[3], hy[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    g.r[k] = P.A[k];
-    g.r[3 + k] = P.B[k];
+    hx[k] = chunk_half(P.X[k], xw1, xu1, xw2, xu2);
+    hy[k] = chunk_half(P.Y[k], yw1, yu1, yw2, yu2);
   }
-  if (pair) {
+  // lo = the own record's half values of chunks 0-2, hi = of chunks 3-5
+  float lo[3], hi[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const uint4 give = sel4(even, P.B[k], P.A[k]);  // what the partner needs
-      const uint4 got = make_uint4(swap_pair(give.x), swap_pair(give.y), swap_pair(give.z), swap_pair(give.w));
-      g.r[k] = sel4(even, P.A[k], got);
-      g.r[3 + k] = sel4(even, got, P.B[k]);
-    }
+  for (int k = 0; k < 3; ++k) {
+    const float send = even ? hy[k] : hx[k];  // what the partner's record needs
+    const float recv = pair ? swapf(send) : hy[k];
+    lo[k] = pair && !even ? recv : hx[k];
+    hi[k] = pair && even ? recv : hy[k];
   }
+  v[0] = lerp1(lo[0], lo[1], w[0], u[0]);
+  v[1] = lerp1(lo[2], hi[0], w[0], u[0]);
+  v[2] = lerp1(hi[1], hi[2], w[0], u[0]);
 }
 
 // Pair gathers while more than cfg.pair_live lanes of the wave are live (32 by default).
@@ -582,11 +676,6 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   __shared__ QBlock blk[7];
   __shared__ float4 x_last[kQuadBlock / 64][kQuadSwitch];
   __shared__ int32_t x_int[kQuadBlock / 64][kQuadSwitch][10];
-  // Cell records: the lane's record of its previous step (chunk-major, so
-  // a wave's 16-B reads of one chunk fall in distinct banks)
-  constexpr bool kReuse = layout_of(S) == PA_LAYOUT_CELLS;
-  constexpr int kChunks = kReuse ? rec_used<S>() : 1;
-  __shared__ uint4 prev_rec[kChunks][kReuse ? kQuadBlock : 1];
   fill_tables<S>(cfg, tab, blk);
   __syncthreads();
 
@@ -637,7 +726,6 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
   __builtin_amdgcn_s_waitcnt(0x0F70);  // the set-up loads, once (see the hand-off below)
   int pend = kDrop;
   float4 pval = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  uint64_t rprev = ~uint64_t(0);  // record address of the lane's previous step
   // byte offset of the lane's slot 0 (slot s is 128 B further per slot)
   const int lbase = static_cast<int>((group8_slot(st.pid, 0, region) - vbase) * 16);
   uint64_t live;
@@ -653,26 +741,16 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     float4 next;
     if constexpr (kCells) {
       float d[3], q[3];
-      const bool inside = (int(locate(f, last.x, last.y, last.z, d, q)) &
+      bool dfast;
+      const bool inside = (int(locate(f, last.x, last.y, last.z, d, q, dfast)) &
                            int(st.state == Q_STEP && st.it != st.R - 1)) != 0;
       int m[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)  // floor(q) for a lane inside the block
-        m[i] = static_cast<int>(fminf(fmaxf(floorf(q[i]), 0.0f), static_cast<float>(f.dims[i] - 2)));
+      for (int i = 0; i < 3; ++i)  // floor(q) for a lane inside the block (a NaN gives 0)
+        m[i] = static_cast<int>(__builtin_amdgcn_fmed3f(floorf(q[i]), 0.0f, f.cell_hi[i]));
       uint64_t r = reinterpret_cast<uint64_t>(cell_record<S>(f, m));
-      const uint64_t rown = r;
-      // While the wave is full (more than cfg.pair_live live lanes, where FP32
-      // records are pair-gathered), a particle still in the cell
-      // of its previous step takes that record from LDS and gathers like a
-      // lane that does not step (no line of its own): share round R = 52
-      // 0.241 -> 0.219 ms; at R = 1002 no change (1.389 -> 1.391 ms), as
-      // the LDS round trip lengthens the long trajectories' steps, so fewer
-      // live lanes skip it.
-      const bool reuse = kReuse && __popcll(live) > cfg.pair_live && inside && r == rprev;
-      const uint64_t W0 = __ballot(inside);
-      const uint64_t W1 = __ballot(inside && !reuse);
-      const uint64_t W = W1 != 0 ? W1 : W0;
-      if (W != 0 && !(inside && !reuse)) {
+      const uint64_t W = __ballot(inside);
+      if (W != 0 && !inside) {
         const int first = __builtin_ctzll(W);
         // readlane returns int: both halves go through uint32_t so that the
         // low half is not sign-extended into the high one
@@ -680,7 +758,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(uint32_t(r)), first));
         r = uint64_t(hi) << 32 | lo;
       }
-      Gather<S> g;
+      [[maybe_unused]] Gather<S> g;
       [[maybe_unused]] PairGather P;
       const bool pair = __popcll(live) > cfg.pair_live;  // wave-uniform
       const bool even = (lane & 1) == 0;
@@ -688,23 +766,21 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       else gather_record<S>(reinterpret_cast<const uint4*>(r), g);
       store16<kStream>(vrs, pval, pend);
       pend = kDrop;
-      // keep the exchange (which waits for the loads) after the store
+      // the weights while the loads are in flight (every lane: a pair's
+      // partner needs them)
+      int mm[3];
+      float wt[3], ut[3], v[3];
+      cell_weights<INTERP>(f, d, q, dfast, mm, wt, ut, !inside);
+      // keep the lerp (which waits for the loads) after the store
       __builtin_amdgcn_sched_barrier(0);
-      if constexpr (storage_of(S) == PA_STORAGE_FP32) pair_finish<S>(P, pair, even, g);
-      if (kReuse && __popcll(live) > cfg.pair_live) {
-#pragma unroll
-        for (int k = 0; k < kChunks; ++k) {
-          g.r[k] = sel4(reuse, prev_rec[k][threadIdx.x], g.r[k]);
-          prev_rec[k][threadIdx.x] = g.r[k];
-        }
-        rprev = inside ? rown : ~uint64_t(0);
-      }
-      if (inside) {
-        float k[24], v[3];
+      if constexpr (storage_of(S) == PA_STORAGE_FP32) {
+        pair_lerp<INTERP>(P, pair, even, wt, ut, v);
+      } else {
+        float k[24];
         gather_corners<S>(g, k);
-        lerp_planar<INTERP>(f, d, q, k, v);
-        a = finish_step<S, MODE, INTERP, STOP_RUNTIME>(cfg, f, last, v, next, rule);
+        lerp_weighted(k, wt, ut, v);
       }
+      if (inside) a = finish_step<S, MODE, INTERP, STOP_RUNTIME>(cfg, f, last, v, next, rule);
     }
     // The segment machine of one iteration, as selects where it can be (a
     // merge of many paths costs register copies in every iteration): START
@@ -833,14 +909,23 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       QGather G;
       if constexpr (kCells) {
         const bool wl = c == 3;
-        G.in = quad_locate(L, wl, ql, G.d, G.q, G.s) & (qs.state == Q_STEP) & (qs.it != qs.R - 1);
-        const float sc = fminf(fmaxf(G.s, 0.0f), L.hi);  // G.s for a quad inside the block
+        G.in = quad_locate(L, wl, ql, G.d, G.q, G.s, G.fast) & (qs.state == Q_STEP) & (qs.it != qs.R - 1);
+        const float sc = __builtin_amdgcn_fmed3f(G.s, 0.0f, L.hi);  // G.s for a quad inside the block (NaN: 0)
         const uint32_t ci = quad_sum(wl ? 0u : static_cast<uint32_t>(static_cast<int>(sc)) * L.stride);
         load_record<S>(L.data, c, ci, G.A, G.B);
       }
       store4<kStream>(vrs, __float_as_int(qval), qpend);
       qpend = kDrop;
-      if (qs.state != Q_IDLE) {
+      if constexpr (kCells && MODE != STEP_CLASSIC) {
+        const bool act = qs.state != Q_IDLE;
+        const float val = quad_iterate_sel<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont,
+                                                            G);
+        qpend = act ? qbase + (qs.slot - 1) * 128 : kDrop;
+        qval = val;
+        const bool isT = quad_all(val == (c == 3 ? 0.0f : -2.0f));
+        qs.pairs += int(act & (qs.slot > 1) & !qs.prevT & !isT);
+        qs.prevT = act ? int(isT) : qs.prevT;
+      } else if (qs.state != Q_IDLE) {
         const float val = quad_iterate<S, MODE, INTERP>(cfg, tab, blk, qs, L, ql, c, rule, region, cont, n_cont,
                                                         kCells ? &G : nullptr);
         qpend = qbase + (qs.slot - 1) * 128;
@@ -848,6 +933,8 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
         const bool isT = quad_all(val == (c == 3 ? 0.0f : -2.0f));
         if (qs.slot > 1 && !qs.prevT && !isT) ++qs.pairs;
         qs.prevT = isT;
+      }
+      {
         if (qs.state == Q_DONE) {
           if (c == 0) {
             counts[qs.pid] = qs.slot;

@@ -1,11 +1,14 @@
-"""The tracer's quotient (p - o) / h without the division sequence, and its
-branch-free fmod, against IEEE division and fmodf on the host.
+"""The tracer's quotient (p - o) / h without the division sequence, its
+remainder from the cell subscript and the reference weight fmod(d, h) / h,
+against IEEE division and fmodf on the host.
 
-The device uses y = RN(1/h), q0 = RN(a y) and two FMA corrections; by
-Markstein's theorem the second is RN(a/h) for a in {0} or [2^-40, 2^40] and
-h in [2^-20, 2^20] (outside, the device divides).  These checks cover whole
-binades of a for the spacings the workloads use and strided sweeps of the
-full range for awkward spacings (all-ones significands, powers of two)."""
+The device holds 1/h as y + yl (y = RN(1/h)), takes q0 = RN(a y + RN(a yl)),
+within one ulp of a/h, and one FMA correction, RN(a/h) by Markstein's
+theorem for a in {0} or [2^-40, 2^40] and h in [2^-20, 2^20] (outside, the
+device divides).  The remainder of a located point is d - m h with m =
+floor(q), plus h when q rounded up to m.  These checks cover whole binades
+of a for the spacings the workloads use and strided sweeps of the full range
+for awkward spacings (all-ones significands, powers of two)."""
 import ctypes as C
 import math
 import os
@@ -23,9 +26,11 @@ def div_lib(tmp_path_factory):
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-shared", "-fPIC", "-o", str(out),
                     os.path.join(HERE, "cpp", "div_check.c"), "-lm"], check=True)
     L = C.CDLL(str(out))
-    for f in (L.quotient_mismatches, L.fmod_mismatches):
+    for f in (L.quotient_mismatches, L.fmod_mismatches, L.weight_mismatches):
         f.restype = C.c_uint64
         f.argtypes = [C.c_float, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.edge_mismatches.restype = C.c_uint64
+    L.edge_mismatches.argtypes = [C.c_float]
     return L
 
 
@@ -51,6 +56,15 @@ def test_quotient_strided_full_range(div_lib, h):
     # a over [2^-40, 2^40] on a stride that visits every significand pattern class
     assert div_lib.quotient_mismatches(float(h), bits(2.0 ** -40), bits(2.0 ** 40), 977) == 0
     assert div_lib.quotient_mismatches(float(h), 0, 1, 1) == 0  # +-0 keeps its sign
+
+
+@pytest.mark.parametrize("h", WORKLOAD_SPACINGS + AWKWARD_SPACINGS, ids=lambda h: "%.6g" % h)
+def test_weight_from_subscript(div_lib, h):
+    # located points d >= 0 with d / h < 2^22: whole binade [2^-3, 2^3) and a strided sweep down to 2^-40
+    hi = min(bits(float(h) * 4194300.0), bits(2.0 ** 40))
+    assert div_lib.weight_mismatches(float(h), bits(0.125), bits(8.0), 1) == 0
+    assert div_lib.weight_mismatches(float(h), bits(2.0 ** -40), hi, 331) == 0
+    assert div_lib.edge_mismatches(float(h)) == 0
 
 
 @pytest.mark.parametrize("h", WORKLOAD_SPACINGS + AWKWARD_SPACINGS[:6], ids=lambda h: "%.6g" % h)
