@@ -126,8 +126,12 @@ __device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, f
   d[0] = px - f.offset[0];
   d[1] = py - f.offset[1];
   d[2] = pz - f.offset[2];
+  // d within [2^-40, 2^40] on every axis (a zero d takes the division, which
+  // gives the same quotient; a NaN d gives a NaN quotient either way);
   // bitwise on purpose: no short-circuit branches
-  fast = (int(f.fast_div != 0) & int(div_range(d[0])) & int(div_range(d[1])) & int(div_range(d[2]))) != 0;
+  const float amin = fminf(fminf(fabsf(d[0]), fabsf(d[1])), fabsf(d[2]));
+  const float amax = fmaxf(fmaxf(fabsf(d[0]), fabsf(d[1])), fabsf(d[2]));
+  fast = (int(f.fast_div != 0) & int(amin >= 0x1p-40f) & int(amax <= 0x1p40f)) != 0;
   if (__builtin_expect(fast, 1)) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) q[i] = div_unsigned(d[i], f.spacing[i], f.inv_spacing[i], f.inv_lo[i]);
@@ -135,11 +139,12 @@ __device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, f
 #pragma unroll
     for (int i = 0; i < 3; ++i) q[i] = d[i] / f.spacing[i];
   }
+  // 0 <= s <= dims - 2 as s == median(s, 0, dims - 2) (false for a NaN s)
   int inside = 1;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float s = floorf(q[i]);
-    inside &= int(s >= 0.0f) & int(s <= f.cell_hi[i]);
+    inside &= int(__builtin_amdgcn_fmed3f(s, 0.0f, f.cell_hi[i]) == s);
   }
   return inside != 0;
 }
@@ -377,12 +382,12 @@ __device__ __forceinline__ float4 step_frozen(const TraceConfig& c, float4 p, co
     o.x = p.x + c.dt * v[0];
     o.y = p.y + c.dt * v[1];
     o.z = p.z + c.dt * v[2];
-    o.w = p.w + c.dt * 0.0f;
+    o.w = p.w + c.w_euler;  // p.w + dt 0
   } else {
     o.x = (((p.x + c.a0 * v[0]) + c.a1 * v[0]) + c.a2 * v[0]) + c.a3 * v[0];
     o.y = (((p.y + c.a0 * v[1]) + c.a1 * v[1]) + c.a2 * v[1]) + c.a3 * v[1];
     o.z = (((p.z + c.a0 * v[2]) + c.a1 * v[2]) + c.a2 * v[2]) + c.a3 * v[2];
-    o.w = (((p.w + c.a0 * 0.0f) + c.a1 * 0.0f) + c.a2 * 0.0f) + c.a3 * 0.0f;
+    o.w = p.w + c.w_rk4;  // (((p.w + a0 0) + a1 0) + a2 0) + a3 0
   }
   return o;
 }
@@ -411,7 +416,7 @@ __device__ __forceinline__ bool step_classic(const TraceConfig& c, const FieldDe
   out.x = (((p.x + c.a0 * k1[0]) + c.a1 * k2[0]) + c.a2 * k3[0]) + c.a3 * k4[0];
   out.y = (((p.y + c.a0 * k1[1]) + c.a1 * k2[1]) + c.a2 * k3[1]) + c.a3 * k4[1];
   out.z = (((p.z + c.a0 * k1[2]) + c.a1 * k2[2]) + c.a2 * k3[2]) + c.a3 * k4[2];
-  out.w = (((p.w + c.a0 * 0.0f) + c.a1 * 0.0f) + c.a2 * 0.0f) + c.a3 * 0.0f;
+  out.w = p.w + c.w_rk4;  // (((p.w + a0 0) + a1 0) + a2 0) + a3 0
   return true;
 }
 

@@ -58,6 +58,10 @@ struct TraceConfig {
   float zero_dt;             // 0 * dt (odeint's zero tableau entries)
   float one_dt;              // 1 * dt
   float stop_eps2;
+  // The w component of a frozen-derivative step, w + a_0 0 + a_1 0 + ..., is
+  // w + w_rk4 (four terms) or w + w_euler (dt 0): +0, -0 if every term is -0,
+  // NaN if one is NaN (zero_terms in trace.hip).
+  float w_rk4, w_euler;
   int32_t interp;
   int32_t xcd_remap;
   int32_t quad_switch;       // hybrid engine: a wave moves to quads at this many live particles (1..16)

@@ -4,6 +4,7 @@
 // each stepped vertex once.
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 #include "pa_device.hpp"
 
@@ -565,6 +566,22 @@ pa_status make_config(pa_gpu_tracer* t, TraceConfig* cfg) {
   return PA_OK;
 }
 
+// The sum w + a_0 0 + a_1 0 + ... (left to right, one rounding each) that a
+// frozen-derivative step takes of the w component equals w + z for every w:
+// each term a_i 0 is +0, -0 (a_i < 0) or NaN (a_i infinite or NaN); adding a
+// signed zero leaves every w but -0 unchanged, and -0 stays -0 only while
+// every term added is -0.  So z = NaN if a term is NaN, -0 if all are -0,
+// else +0.
+static float zero_terms(const float* a, int n) {
+  bool all_neg = true;
+  for (int i = 0; i < n; ++i) {
+    const float t = a[i] * 0.0f;
+    if (t != t) return std::numeric_limits<float>::quiet_NaN();
+    if (!std::signbit(t)) all_neg = false;
+  }
+  return all_neg ? -0.0f : 0.0f;
+}
+
 void fill_step_constants(const pa_gpu_tracer* t, TraceConfig* cfg) {
   const float dt = t->step_size;
   // odeint runge_kutta4 tableau in value_type float: b = {1/6, 1/3, 1/3, 1/6}.
@@ -578,6 +595,9 @@ void fill_step_constants(const pa_gpu_tracer* t, TraceConfig* cfg) {
   cfg->zero_dt = 0.0f * dt;
   cfg->one_dt = 1.0f * dt;
   cfg->stop_eps2 = t->stop_eps * t->stop_eps;
+  const float rk4[4] = {cfg->a0, cfg->a1, cfg->a2, cfg->a3};
+  cfg->w_rk4 = zero_terms(rk4, 4);
+  cfg->w_euler = zero_terms(&cfg->dt, 1);
   cfg->interp = t->interp;
   cfg->xcd_remap = t->xcd_remap;
   cfg->quad_switch = t->quad_switch;

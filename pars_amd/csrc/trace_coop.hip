@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kWave) void k_coop(const TraceConfig cfg, const Coo
           o.x = (((last.x + cfg.a0 * k1[0]) + cfg.a1 * k2[0]) + cfg.a2 * k3[0]) + cfg.a3 * vel[0];
           o.y = (((last.y + cfg.a0 * k1[1]) + cfg.a1 * k2[1]) + cfg.a2 * k3[1]) + cfg.a3 * vel[1];
           o.z = (((last.z + cfg.a0 * k1[2]) + cfg.a1 * k2[2]) + cfg.a2 * k3[2]) + cfg.a3 * vel[2];
-          o.w = (((last.w + cfg.a0 * 0.0f) + cfg.a1 * 0.0f) + cfg.a2 * 0.0f) + cfg.a3 * 0.0f;
+          o.w = last.w + cfg.w_rk4;  // (((last.w + a0 0) + a1 0) + a2 0) + a3 0
           stage = 0;
           emit(o);
         }

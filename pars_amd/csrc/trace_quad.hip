@@ -620,7 +620,7 @@ __device__ __forceinline__ void pair_lerp(const PairGather& P, bool pair, bool e
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float send = even ? hy[k] : hx[k];  // what the partner's record needs
-    const float recv = pair ? swapf(send) : hy[k];
+    const float recv = swapf(send);  // used only in pair mode
     lo[k] = pair && !even ? recv : hx[k];
     hi[k] = pair && even ? recv : hy[k];
   }
