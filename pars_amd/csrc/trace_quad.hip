@@ -156,8 +156,9 @@ __device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, flo
 // the lane's axis: d = x - offset, q = d / h, s = floor(q); quad-uniform
 // result, false = outside the block.
 // fast: d and the spacing were within div_corrected's ranges (or the w lane).
+// own: the lane's own axis is inside (0 <= s <= dims - 2; always on the w lane).
 __device__ __forceinline__ bool quad_locate(const QLane& L, bool wl, float x, float& d, float& q, float& s,
-                                            bool& fast) {
+                                            bool& fast, bool& own) {
   d = x - L.off;
   // per lane: both branches give the correctly rounded quotient
   fast = (int(wl | div_range(d)) & int(L.fast != 0)) != 0;
@@ -165,7 +166,8 @@ __device__ __forceinline__ bool quad_locate(const QLane& L, bool wl, float x, fl
   if (__builtin_expect(fast, 1)) q = div_unsigned(d, L.h, L.inv, L.inv_lo);
   else q = d / L.h;
   s = floorf(q);
-  return quad_all(wl | ((s >= 0.0f) & (s <= L.hi)));
+  own = wl | (__builtin_amdgcn_fmed3f(s, 0.0f, L.hi) == s);
+  return quad_all(own);
 }
 
 // contains + interpolate of vector_field.cpp:12-48 at the quad's point
@@ -175,8 +177,8 @@ template <int S, int INTERP>
 __device__ __forceinline__ bool quad_eval(const QLane& L, int c, float x, float& v) {
   const bool wl = c == 3;
   float d, q, s;
-  bool dfast;
-  if (!quad_locate(L, wl, x, d, q, s, dfast)) return false;
+  bool dfast, own;
+  if (!quad_locate(L, wl, x, d, q, s, dfast, own)) return false;
   const uint32_t m = wl ? 0u : static_cast<uint32_t>(static_cast<int>(s));
   float k[8];  // component c of c000 c001 c010 c011 c100 c101 c110 c111
   if constexpr (layout_of(S) == PA_LAYOUT_CELLS) {
@@ -742,12 +744,12 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
     if constexpr (kCells) {
       float d[3], q[3];
       bool dfast;
-      const bool inside = (int(locate(f, last.x, last.y, last.z, d, q, dfast)) &
-                           int(st.state == Q_STEP && st.it != st.R - 1)) != 0;
+      const bool located = locate(f, last.x, last.y, last.z, d, q, dfast);
+      const bool inside = (int(located) & int(st.state == Q_STEP && st.it != st.R - 1)) != 0;
       int m[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)  // floor(q) for a lane inside the block (a NaN gives 0)
-        m[i] = static_cast<int>(__builtin_amdgcn_fmed3f(floorf(q[i]), 0.0f, f.cell_hi[i]));
+      for (int i = 0; i < 3; ++i)  // floor(q), within the block, for a located lane; cell 0 for the others
+        m[i] = located ? static_cast<int>(floorf(q[i])) : 0;
       uint64_t r = reinterpret_cast<uint64_t>(cell_record<S>(f, m));
       const uint64_t W = __ballot(inside);
       if (W != 0 && !inside) {
@@ -909,9 +911,10 @@ __global__ __launch_bounds__(kQuadBlock) void k_hybrid_union(const TraceConfig c
       QGather G;
       if constexpr (kCells) {
         const bool wl = c == 3;
-        G.in = quad_locate(L, wl, ql, G.d, G.q, G.s, G.fast) & (qs.state == Q_STEP) & (qs.it != qs.R - 1);
-        const float sc = __builtin_amdgcn_fmed3f(G.s, 0.0f, L.hi);  // G.s for a quad inside the block (NaN: 0)
-        const uint32_t ci = quad_sum(wl ? 0u : static_cast<uint32_t>(static_cast<int>(sc)) * L.stride);
+        bool own;
+        G.in = quad_locate(L, wl, ql, G.d, G.q, G.s, G.fast, own) & (qs.state == Q_STEP) & (qs.it != qs.R - 1);
+        // the lane's subscript where its axis is inside, else 0: a valid cell for every quad
+        const uint32_t ci = quad_sum(wl || !own ? 0u : static_cast<uint32_t>(static_cast<int>(G.s)) * L.stride);
         load_record<S>(L.data, c, ci, G.A, G.B);
       }
       store4<kStream>(vrs, __float_as_int(qval), qpend);
