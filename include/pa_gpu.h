@@ -291,9 +291,9 @@ pa_status   pa_gpu_tracer_set_engine(pa_gpu_tracer* tracer, pa_engine engine, in
 pa_status   pa_gpu_tracer_set_xcd_remap(pa_gpu_tracer* tracer, int32_t enable);
 /* Extension: launch constants of the hybrid engine, for an autotuner: a
  * wave moves its live particles onto quads once at most quad_switch (1..16,
- * default 16) are left; FP32 cell records are gathered by lane pairs (and
- * kept in LDS for reuse) while more than pair_live (0..64, default 32)
- * lanes are live.  Results are unchanged; only the speed is. */
+ * default 16) are left; FP32 cell records are gathered by lane pairs while
+ * more than pair_live (0..64, default 16) lanes are live.  Results are
+ * unchanged; only the speed is. */
 pa_status   pa_gpu_tracer_set_launch(pa_gpu_tracer* tracer, int32_t quad_switch, int32_t pair_live);
 /* Union-of-7 topology: bit d of face_mask[k] set when neighbour block k has
  * its own neighbour across face d (partitioner.cpp:47-52 order). */

@@ -412,7 +412,7 @@ class particle_tracer:
         """XCD workgroup order: 0 hardware order, 1 contiguous eighths, 2..64 runs of that many workgroups."""
         self.ctx.check(lib().pa_gpu_tracer_set_xcd_remap(self._h, C.c_int32(int(order))), "set_xcd_remap")
 
-    def set_launch(self, quad_switch: int = 16, pair_live: int = 32):
+    def set_launch(self, quad_switch: int = 16, pair_live: int = 16):
         """Hybrid engine launch constants (pa_gpu_tracer_set_launch); results do not depend on them."""
         self.ctx.check(lib().pa_gpu_tracer_set_launch(self._h, C.c_int32(quad_switch), C.c_int32(pair_live)),
                        "set_launch")

@@ -129,7 +129,7 @@ struct pa_gpu_tracer {
   int32_t waves = 0;  // resident waves of the cooperative engine (0 = automatic)
   int32_t xcd_remap = 1;
   int32_t quad_switch = 16;  // launch constants of the hybrid engine (pa_gpu_tracer_set_launch)
-  int32_t pair_live = 32;
+  int32_t pair_live = 16;
 };
 
 namespace pa_amd {

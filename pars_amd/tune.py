@@ -22,7 +22,7 @@ import numpy as np
 from . import ROUND_RESULT_DTYPE, Context, DeviceBuffer, Particles, particle_tracer
 
 QUAD_SWITCH = (8, 12, 16)
-PAIR_LIVE = (16, 32, 48)
+PAIR_LIVE = (0, 16, 32)
 XCD_REMAP = (0, 1, 2, 4, 8)
 
 
@@ -89,7 +89,7 @@ def tune_round(ctx: Context, tracer: particle_tracer, seeds: Particles, region_s
         table[_label(q, p, x)] = ms
         if best is None or ms < best[3]:
             best = (q, p, x, ms)
-    default = table.get(_label(16, 32, 1), best[3])
+    default = table.get(_label(16, 16, 1), best[3])  # the tracer's defaults
     tun = RoundTuning(best[0], best[1], best[2], best[3], default, table)
     tun.apply(tracer)
     if cache and key:
