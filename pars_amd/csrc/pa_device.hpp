@@ -132,12 +132,15 @@ __device__ __forceinline__ bool locate(const FieldDesc& f, float px, float py, f
   const float amin = fminf(fminf(fabsf(d[0]), fabsf(d[1])), fabsf(d[2]));
   const float amax = fmaxf(fmaxf(fabsf(d[0]), fabsf(d[1])), fabsf(d[2]));
   fast = (int(f.fast_div != 0) & int(amin >= 0x1p-40f) & int(amax <= 0x1p40f)) != 0;
-  if (__builtin_expect(fast, 1)) {
+  // the corrected quotient everywhere; the division only where some lane of
+  // the wave needs it (a wave-uniform branch, not an if/else per lane)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) q[i] = div_unsigned(d[i], f.spacing[i], f.inv_spacing[i], f.inv_lo[i]);
-  } else {
+  for (int i = 0; i < 3; ++i) q[i] = div_unsigned(d[i], f.spacing[i], f.inv_spacing[i], f.inv_lo[i]);
+  if (__builtin_expect(__ballot(!fast) != 0, 0)) {
+    if (!fast) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) q[i] = d[i] / f.spacing[i];
+      for (int i = 0; i < 3; ++i) q[i] = d[i] / f.spacing[i];
+    }
   }
   // 0 <= s <= dims - 2 as s == median(s, 0, dims - 2) (false for a NaN s)
   int inside = 1;
@@ -179,12 +182,14 @@ __device__ __forceinline__ void cell_weights(const FieldDesc& f, const float d[3
 #pragma unroll
     for (int i = 0; i < 3; ++i) r[i] = rem_located(d[i], f.spacing[i], fl[i]);
     const uint32_t key = min(min(rem_key(r[0]), rem_key(r[1])), rem_key(r[2]));
-    if (__builtin_expect((int(unused) | (int(dfast) & int(key >= kRemKeyMin))) != 0, 1)) {
+    const bool fast = (int(unused) | (int(dfast) & int(key >= kRemKeyMin))) != 0;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) w[i] = weight_located(d[i], f.spacing[i], f.inv_spacing[i], f.inv_lo[i], r[i]);
-    } else {
+    for (int i = 0; i < 3; ++i) w[i] = weight_located(d[i], f.spacing[i], f.inv_spacing[i], f.inv_lo[i], r[i]);
+    if (__builtin_expect(__ballot(!fast) != 0, 0)) {  // as in locate
+      if (!fast) {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) w[i] = exact_fmod(d[i], f.spacing[i], q[i]) / f.spacing[i];
+        for (int i = 0; i < 3; ++i) w[i] = exact_fmod(d[i], f.spacing[i], q[i]) / f.spacing[i];
+      }
     }
   }
 #pragma unroll

@@ -139,10 +139,11 @@ __device__ __forceinline__ float quad_lerp(const QLane& L, bool wl, float d, flo
     // as cell_weights: the remainder from s = floor(q) while d and it are
     // within div_corrected's ranges, else fmodf and division
     const float r = rem_located(d, L.h, s);
-    if (__builtin_expect((int(unused) | (int(dfast) & int(rem_range(r)))) != 0, 1))
-      w = weight_located(d, L.h, L.inv, L.inv_lo, r);
-    else
-      w = exact_fmod(d, L.h, q) / L.h;
+    const bool fast = (int(unused) | (int(dfast) & int(rem_range(r)))) != 0;
+    w = weight_located(d, L.h, L.inv, L.inv_lo, r);
+    if (__builtin_expect(__ballot(!fast) != 0, 0)) {  // as in locate
+      if (!fast) w = exact_fmod(d, L.h, q) / L.h;
+    }
   }
   const float w0 = qmovf<kBcast0>(w), w1 = qmovf<kBcast1>(w), w2 = qmovf<kBcast2>(w);
   const float u0 = 1.0f - w0, u1 = 1.0f - w1, u2 = 1.0f - w2;
@@ -162,9 +163,12 @@ __device__ __forceinline__ bool quad_locate(const QLane& L, bool wl, float x, fl
   d = x - L.off;
   // per lane: both branches give the correctly rounded quotient
   fast = (int(wl | div_range(d)) & int(L.fast != 0)) != 0;
-  // (a zero quotient may carry either sign, as in locate)
-  if (__builtin_expect(fast, 1)) q = div_unsigned(d, L.h, L.inv, L.inv_lo);
-  else q = d / L.h;
+  // (a zero quotient may carry either sign; the division under a
+  // wave-uniform branch, as in locate)
+  q = div_unsigned(d, L.h, L.inv, L.inv_lo);
+  if (__builtin_expect(__ballot(!fast) != 0, 0)) {
+    if (!fast) q = d / L.h;
+  }
   s = floorf(q);
   own = wl | (__builtin_amdgcn_fmed3f(s, 0.0f, L.hi) == s);
   return quad_all(own);
