@@ -530,6 +530,28 @@ def bench_ftle(pa, ctx, orc, n=256, steps=1000, dt=0.01, forms=FTLE_FORMS):
     return head
 
 
+def scattered_line_rate(pa, ctx, nbytes, line=64, read=48, n_reads=1 << 30):
+    """Reads per second of pa_gpu_gather_lines: read-byte reads at the start of
+    pseudo-random line-byte lines of an nbytes buffer, 8 in flight per lane."""
+    lib = pa.lib()
+    src, out = ctx.alloc(nbytes), ctx.alloc(16)
+
+    def run(n, seed):
+        ctx.check(lib.pa_gpu_gather_lines(ctx.handle, C.c_void_p(src.ptr), C.c_size_t(nbytes), C.c_int32(line),
+                                          C.c_int32(read), C.c_int64(n), C.c_uint64(seed), C.c_void_p(out.ptr)),
+                  "gather_lines")
+
+    run(n_reads // 8, 1)  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    run(n_reads, 2)
+    ctx.synchronize()
+    sec = time.perf_counter() - t0
+    src.free()
+    out.free()
+    return n_reads / sec
+
+
 def bench_batched(pa, ctx, cache=None):
     """Batched uneven work: 8 random-Fourier curl fields (64^3 .. 256^3, BF16),
     64 seed sets (log-uniform 1e3..1e6 seeds, 100..2000 steps, dt 0.01) in one
@@ -586,6 +608,13 @@ def bench_batched(pa, ctx, cache=None):
                        "index, colour per chunk of <= 134,217,727 vertices), then discarded")
     for f in fields:
         f.release()
+    # the rate of unrelated-line reads over the same footprint (64-B cell records), what the pool's
+    # cell changes cost once they miss the caches
+    footprint = sum((g - 1) ** 3 for g in wl.grids) * 64
+    scattered = dict(footprint_bytes=footprint, line_bytes=64, read_bytes=48,
+                     reads_per_s=scattered_line_rate(pa, ctx, footprint),
+                     note="pa_gpu_gather_lines: 48-B reads of pseudo-random 64-B records over the batch's "
+                          "cell-record footprint, 8 in flight per lane")
     gather = 48  # BF16: 8 corners x 3 components x 2 B
     return dict(workload="8 curl fields of 48 random modes (64^3..256^3 nodes, BF16), 64 seed sets "
                          "(%d seeds), 100..2000 RK4 steps per set, dt 0.01, one persistent launch" % wl.n,
@@ -596,6 +625,7 @@ def bench_batched(pa, ctx, cache=None):
                               algorithmic_bytes_per_step=gather, achieved_gbs=steps * gather / (kernel_ms * 1e-3) / 1e9,
                               peak=HBM_PEAK_GBS, frac=steps * gather / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
                 graph=dict(seconds_per_round=gsec, steps_per_s=gsteps / gsec, same_results=graph_same),
+                scattered_lines=scattered,
                 curves=curves,
                 tuning=dict(layout="cells" if tun.layout == pa.LAYOUT_CELLS else "nodes",
                             waves_per_cu=tun.waves_per_cu, sample_rates=tun.table))

@@ -151,6 +151,15 @@ pa_status   pa_gpu_memcpy_d2d(pa_gpu_ctx* ctx, void* d_dst, const void* d_src, s
  * to calibrate hardware byte counters against a known transfer. */
 pa_status   pa_gpu_copy(pa_gpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 
+/* Scattered line reads by a plain kernel (calibration, no counterpart in the
+ * reference): n_reads reads of read_bytes (16 to 64, 16-byte units) at the
+ * start of pseudo-random line_bytes lines of [d_src, d_src + bytes), 8 in
+ * flight per lane; the XOR of every 32-bit word read is XORed into *d_xor.
+ * Its rate bounds kernels whose gathers land on unrelated lines (the
+ * particle pool's). */
+pa_status   pa_gpu_gather_lines(pa_gpu_ctx* ctx, const void* d_src, size_t bytes, int32_t line_bytes,
+                                int32_t read_bytes, int64_t n_reads, uint64_t seed, uint32_t* d_xor);
+
 /* Kernel timing: when enabled, every kernel launch of the library is
  * bracketed with HIP events on the context stream; per-kernel totals are
  * read with pa_gpu_kernel_time (name as in the kernel table, e.g.
@@ -497,8 +506,8 @@ pa_status   pa_gpu_pool_init(pa_gpu_ctx* ctx, const pa_particle* d_particles, si
                              const pa_pool* pool);
 /* Advance every active entry of [0, n) by at most max_iterations reference
  * iterations (0 = to the end) with the tracer's integrator, step size, stop
- * rule and interpolation; waves_per_cu = resident waves of the persistent
- * kernel per CU (0 = 16).  All fields share one storage and layout.  An
+ * rule and interpolation; waves_per_cu = waves of the persistent kernel
+ * per CU (0 = as many as one CU keeps resident, 12 to 20 by variant).  All fields share one storage and layout.  An
  * active entry whose field index is n_fields or more becomes
  * PA_POOL_INVALID without being stepped and the call returns PA_E_INVALID
  * once the others have run. */

@@ -259,7 +259,7 @@ def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField
     to its end; pa_gpu_pool_run_vertices: each particle writes its one-block
     reference slots into its set's part of the arena).  Bounded passes hand
     sets out sooner but end each pass in a latency-bound tail (on the batched
-    workload: passes of 400 iterations 237 ms of pool time, to the end 127
+    workload: passes of 400 iterations 118 ms of pool time, to the end 105
     ms).  After each pass, a set whose particles have all ended is finished on the device
     in chunks of at most chunk_vertices slots, as the reference chunks its
     curves (chunk = chunk_vertices // R seeds): counted prune, index, colour;
@@ -361,3 +361,41 @@ def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField
     tot["seconds"] = time.perf_counter() - t0
     tot["arena_slots"] = int(arena_slots)
     return tot
+
+
+def main(argv=None):
+    """python -m pars_amd.batch: the batched workload on device 0 (BF16 cell
+    records): `--runs` timed persistent launches, and with `--curves` the same
+    batch with every set's curves at each of `--pass-iterations` (0: each pass
+    to the end), one JSON line per run."""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n")[0])
+    ap.add_argument("--runs", type=int, default=3)
+    ap.add_argument("--waves", type=int, default=0, help="waves per CU (0: as many as stay resident)")
+    ap.add_argument("--curves", action="store_true")
+    ap.add_argument("--pass-iterations", type=int, nargs="*", default=[0, 400])
+    args = ap.parse_args(argv)
+    ctx = Context(0)
+    wl = make_workload()
+    fields = build_fields(ctx, wl, BF16, LAYOUT_CELLS)
+    steps, sec, pool = run_batch(ctx, wl, fields, args.waves)  # warm-up
+    for r in range(args.runs):
+        steps, sec, pool = run_batch(ctx, wl, fields, args.waves, pool=pool)
+        ctx.synchronize()
+        print(json.dumps(dict(run=r, particles=wl.n, steps=steps, seconds=sec, steps_per_s=steps / sec)), flush=True)
+    if args.curves:
+        for it in args.pass_iterations:
+            ctx.timing_reset()
+            ctx.timing(True)
+            tot = run_batch_vertices(ctx, wl, fields, args.waves, pass_iterations=it)
+            ctx.synchronize()
+            ctx.timing(False)
+            kt = ctx.kernel_time("pool_run")
+            print(json.dumps(dict(pass_iterations=it, pool_run_ms=kt[0], **tot)), flush=True)
+    for f in fields:
+        f.release()
+
+
+if __name__ == "__main__":
+    main()

@@ -22,6 +22,7 @@ namespace {
 constexpr int kPoolBlock = 256;
 constexpr int kStage = 4;  // vertices a lane stages in LDS before writing them
 constexpr int kMaxFields = 16;
+constexpr int kChunk = 64;  // pool entries a wave claims at a time
 
 struct PoolConfig {
   TraceConfig k;                // integrator constants, stop rule, interpolation
@@ -62,86 +63,158 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_init(const pa_particle* __r
   if (!ok) atomicAdd(invalid, 1u);
 }
 
-// Persistent: lanes take pool entries from a work counter (one atomic per
-// wave per refill), advance each for at most `budget` iterations (or to the
-// end), store the state back and take the next.  Finished entries add one to
-// their request's counter.
+// Whether a pool kernel keeps each lane's last cell record in LDS: 16-bit
+// cell records (three uint4) with one field evaluation per step.
+template <int S, int MODE>
+constexpr bool keeps_record() {
+  return layout_of(S) == PA_LAYOUT_CELLS && storage_of(S) != PA_STORAGE_FP32 && MODE != STEP_CLASSIC;
+}
+
+// advance_rt() for a pool lane that keeps its last cell record in LDS
+// (keep[0..2][lane], cell index `kept`): a step that stays in the kept cell
+// reads the record from LDS, any other step gathers it and keeps it.  The
+// pool's lanes follow unrelated particles, so every gather is a miss of its
+// own in the vector L1, whose outstanding misses bound the kernel; most steps
+// stay in their cell and now make none.  The values are those of the gather.
 template <int S, int MODE, int INTERP>
-__global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
+__device__ __forceinline__ int advance_kept(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next,
+                                            int rule, uint32_t& kept, uint4 (*keep)[kPoolBlock]) {
+  float d[3], q[3], v[3];
+  if (!locate(f, last.x, last.y, last.z, d, q)) return 1;
+  int m[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) m[i] = static_cast<int>(floorf(q[i]));
+  // fewer than 2^31 records (checked when the layout is built), so no cell has index 0xffffffff
+  const uint32_t ci = uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
+  const bool hit = ci == kept;
+  Gather<S> g;
+  if (hit) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g.r[k] = keep[k][threadIdx.x];
+  } else {
+    const uint4* r = static_cast<const uint4*>(f.cells) + uint64_t(ci) * rec_size<S>();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g.r[k] = gload<uint4>(r, k);
+  }
+  float k[24];
+  gather_corners<S>(g, k);
+  lerp_planar<INTERP>(f, d, q, k, v);
+  if (!hit) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) keep[j][threadIdx.x] = g.r[j];
+    kept = ci;
+  }
+  return finish_step<S, MODE, INTERP, STOP_RUNTIME>(c, f, last, v, next, rule);
+}
+
+// Persistent: each wave takes pool entries in chunks of `chunk` from a work
+// counter, the claim of its next chunk always in flight behind the steps; a
+// lane that has no entry takes the next of its wave's chunk, whose state
+// loads go out with the other lanes' gathers and land at the same wait, and
+// starts stepping it in the following iteration.  A lane advances an entry
+// for at most `budget` iterations (or to the end), stores the state back and
+// takes the next.  Finished entries add one to their request's counter.
+// Entries go to lanes in no fixed order; a particle's result does not depend
+// on it.
+template <int S, int MODE, int INTERP>
+__global__ __launch_bounds__(kPoolBlock) __attribute__((amdgpu_waves_per_eu(keeps_record<S, MODE>() ? 5 : 1)))
+void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
                                                          int32_t* __restrict__ it_a, const int32_t* __restrict__ rem,
                                                          int32_t* __restrict__ steps_a,
                                                          int32_t* __restrict__ status_a,
                                                          const int32_t* __restrict__ field_a,
                                                          const int32_t* __restrict__ request,
                                                          int32_t* __restrict__ request_done, int n_requests,
-                                                         int64_t n, int budget, int rule,
+                                                         int64_t n, int budget, int rule, int chunk,
                                                          unsigned long long* __restrict__ work,
                                                          unsigned long long* __restrict__ steps_out,
                                                          unsigned long long* __restrict__ rejected,
                                                          const PoolVertices V) {
+  // The field table in LDS: a lane reads its entry's field by index when it takes the entry.
+  static_assert(sizeof(FieldDesc) % 4 == 0, "FieldDesc copies as 32-bit words");
+  __shared__ FieldDesc ftab[kMaxFields];
+  {
+    constexpr int kWords = int(sizeof(FieldDesc)) * kMaxFields / 4;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(cfg.f);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(ftab);
+    for (int k = threadIdx.x; k < kWords; k += kPoolBlock) dst[k] = src[k];
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
-  bool have = false, open = true;
-  int64_t i = 0, vb = 0;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  const unsigned long long un = static_cast<unsigned long long>(n);
+  const unsigned long long csize = static_cast<unsigned long long>(chunk);
+  // The wave's chunk [cur, end); `ahead` (lane 0) is the start of the chunk claimed next, valid while
+  // `claimed` (the claim went out before any chunk reached n).
+  unsigned long long cur = 0, end = 0, ahead = 0;
+  bool claimed = true;
+  if (lane == 0) ahead = atomicAdd(work, csize);
   // Vertex output is staged per lane in LDS and written kStage slots at a time: a global store counts
   // in the same in-order wait as the next gather, so one per step would lengthen every step.
   __shared__ float4 stage[kStage][kPoolBlock];
+  constexpr bool kKeep = keeps_record<S, MODE>();
+  __shared__ uint4 keep[kKeep ? 3 : 1][kPoolBlock];  // the lane's last cell record (kKeep)
+  uint32_t kept = 0xffffffffu;                        // its cell index; none
   int staged = 0;       // vertices waiting in LDS
   int64_t sfirst = 0;   // slot of the first of them
+  bool have = false;    // the lane is stepping entry i
+  bool pend = false;    // the lane's next entry is loading (the t* values)
+  int64_t i = 0, vb = 0;
   float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   int it = 0, R = 0, st = 0, left = 0;
-  FieldDesc f = cfg.f[0];
+  int fi = 0;  // the entry's field: ftab[fi], read from LDS where a step uses it
+  int64_t ti = 0, tvb = 0;
+  float4 tp = p;
+  int32_t tstat = 0, tfi = 0, trem = 0, tit = 0, tst = 0;
   unsigned long long done_steps = 0;
   for (;;) {
-    const uint64_t idle = __ballot(!have);
-    if (open && idle) {
-      const uint32_t take = uint32_t(__popcll(idle));
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(work, static_cast<unsigned long long>(take));
-      const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(base & 0xffffffffu));
-      const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(base >> 32));
-      base = (static_cast<unsigned long long>(hi) << 32) | lo;
-      if (base + take >= static_cast<unsigned long long>(n)) open = false;
-      if (!have) {
-        const unsigned long long idx = base + uint32_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
-        if (idx < static_cast<unsigned long long>(n) && status_a[idx] == PA_POOL_ACTIVE) {
-          const int32_t fi = field_a[idx];
-          const int32_t r_idx = rem[idx];
-          const int64_t b_idx = V.slots ? V.base[idx] : 0;
-          if (fi < 0 || fi >= cfg.n_fields ||
-              (V.slots && (b_idx < 0 || b_idx > V.capacity - (r_idx > 1 ? r_idx : 1)))) {
-            // a field this launch was not given, or slots outside the vertex buffer: never stepped
-            status_a[idx] = PA_POOL_INVALID;
-            atomicAdd(rejected, 1ull);
-          } else {
-            i = int64_t(idx);
-            p = pos[i];
-            it = it_a[i];
-            R = r_idx;
-            st = steps_a[i];
-            f = cfg.f[fi];
-            left = budget;
-            have = true;
-            vb = b_idx;
-            staged = 0;
-            sfirst = vb + it;
-            if (V.slots && it == 1) {  // first load: the seed (initialize, :243-252)
-              stage[0][threadIdx.x] = p;
-              staged = 1;
-              sfirst = vb;
-            }
-          }
+    // hand out entries to the lanes that have none
+    const uint64_t idle = __ballot(!have && !pend);
+    if (idle && (cur < end || claimed)) {
+      const uint32_t want = uint32_t(__popcll(idle));
+      const uint32_t rank = uint32_t(__popcll(idle & below));
+      const unsigned long long avail = end - cur;
+      unsigned long long idx = cur + rank, lim = end;
+      if (want <= avail) {
+        cur += want;
+      } else if (claimed) {
+        // the current chunk runs out: continue in the one claimed ahead and claim the next
+        const unsigned long long nb = uniform(int64_t(ahead));
+        const unsigned long long ne = nb < un ? (un - nb > csize ? nb + csize : un) : nb;
+        if (rank >= avail) {
+          idx = nb + (rank - avail);
+          lim = ne;
         }
+        cur = nb + (want - avail) < ne ? nb + (want - avail) : ne;
+        end = ne;
+        claimed = ne < un;
+        if (claimed && lane == 0) ahead = atomicAdd(work, csize);
+      } else {
+        cur = end;
+      }
+      if (!have && !pend && idx < lim) {
+        ti = int64_t(idx);
+        tstat = status_a[ti];
+        tfi = field_a[ti];
+        trem = rem[ti];
+        tp = pos[ti];
+        tit = it_a[ti];
+        tst = steps_a[ti];
+        tvb = V.slots ? V.base[ti] : 0;
+        pend = true;
       }
     }
-    if (__ballot(have) == 0) {
-      if (!open) break;
-      continue;
-    }
+    if (__ballot(have || pend) == 0) break;  // nothing held and nothing left to take
     if (have) {
       bool fin = it >= R - 1;  // the reference breaks at it == R-1 (R < 2: the seed alone)
       if (!fin) {
         float4 next;
-        if (advance_rt<S, MODE, INTERP>(cfg.k, f, p, next, rule) != 0) {
+        int out;
+        if constexpr (kKeep)
+          out = advance_kept<S, MODE, INTERP>(cfg.k, ftab[fi], p, next, rule, kept, keep);
+        else
+          out = advance_rt<S, MODE, INTERP>(cfg.k, ftab[fi], p, next, rule);
+        if (out != 0) {
           fin = true;
         } else {
           if (V.slots) {
@@ -179,6 +252,35 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_run(const PoolConfig cfg, f
           }
         }
         have = false;
+      }
+    }
+    // a lane whose entry has loaded takes it up: it steps from the next iteration
+    if (pend) {
+      pend = false;
+      if (tstat == PA_POOL_ACTIVE) {
+        if (tfi < 0 || tfi >= cfg.n_fields || (V.slots && (tvb < 0 || tvb > V.capacity - (trem > 1 ? trem : 1)))) {
+          // a field this launch was not given, or slots outside the vertex buffer: never stepped
+          status_a[ti] = PA_POOL_INVALID;
+          atomicAdd(rejected, 1ull);
+        } else {
+          i = ti;
+          p = tp;
+          it = tit;
+          R = trem;
+          st = tst;
+          fi = tfi;
+          kept = 0xffffffffu;
+          left = budget;
+          have = true;
+          vb = tvb;
+          staged = 0;
+          sfirst = vb + it;
+          if (V.slots && it == 1) {  // first load: the seed (initialize, :243-252)
+            stage[0][threadIdx.x] = p;
+            staged = 1;
+            sfirst = vb;
+          }
+        }
       }
     }
   }
@@ -284,22 +386,27 @@ pa_status pool_launch(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32
   auto* counters = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->counters) + 2048);
   int cus = 0;
   PA_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  const int per_cu = waves_per_cu > 0 ? waves_per_cu : 16;
-  int64_t blocks = (int64_t(cus) * per_cu + 3) / 4;
   const int64_t need = (int64_t(n) + kPoolBlock - 1) / kPoolBlock;
-  if (blocks > need) blocks = need;
-  if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_pool_zero, dim3(1), dim3(64), 0, ctx->stream, counters);
   const int budget = max_iterations > 0 ? max_iterations : INT32_MAX;
+  hipError_t occ = hipSuccess;
   launch_begin(ctx, "pool_run");
   dispatch(cfg.f[0].storage, mode_of_integrator(t->integrator), cfg.k.interp, [&](auto S, auto M, auto I) {
-    hipLaunchKernelGGL((k_pool_run<decltype(S)::value, decltype(M)::value, decltype(I)::value>),
-                       dim3(unsigned(blocks)), dim3(kPoolBlock), 0, ctx->stream, cfg,
+    const auto kernel = k_pool_run<decltype(S)::value, decltype(M)::value, decltype(I)::value>;
+    // waves_per_cu 0: as many workgroups per CU as the kernel keeps resident
+    int per_cu_blocks = 0;
+    if (waves_per_cu == 0) occ = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_blocks, kernel, kPoolBlock, 0);
+    int64_t blocks = waves_per_cu > 0 ? (int64_t(cus) * waves_per_cu + 3) / 4
+                                      : int64_t(cus) * (per_cu_blocks > 0 ? per_cu_blocks : 4);
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(kernel, dim3(unsigned(blocks)), dim3(kPoolBlock), 0, ctx->stream, cfg,
                        reinterpret_cast<float4*>(pool->position), pool->iteration, pool->remaining, pool->steps,
                        pool->status, pool->field, pool->request, pool->request_done, int(pool->n_requests),
-                       int64_t(n), budget, t->stop_rule, counters, counters + 1, counters + 2, V);
+                       int64_t(n), budget, t->stop_rule, kChunk, counters, counters + 1, counters + 2, V);
   }, cfg.f[0].layout);
   launch_end(ctx);
+  PA_HIP(ctx, occ);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;
 }

@@ -272,6 +272,65 @@ pa_status pa_gpu_copy(pa_gpu_ctx* ctx, void* d_dst, const void* d_src, size_t by
   return PA_OK;
 }
 
+namespace {
+// Scattered line reads (calibration): read j (0 <= j < n_reads) takes
+// chunk_u4 uint4 from the start of line (mix(j ^ seed) * n_lines) >> 32,
+// lines of line_u4 uint4; each lane keeps 8 reads in flight.  The XOR of
+// everything read goes to *out (one atomic per wave), so no read is dead.
+__device__ __forceinline__ uint32_t mix32(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return uint32_t(z ^ (z >> 31));
+}
+__global__ __launch_bounds__(256) void k_gather_lines(const uint4* __restrict__ src, uint32_t n_lines, int line_u4,
+                                                      int chunk_u4, int64_t n_reads, uint64_t seed,
+                                                      unsigned* __restrict__ out) {
+  uint32_t acc = 0;
+  const int64_t stride = int64_t(gridDim.x) * 256;
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n_reads; i += 8 * stride) {
+    uint4 v[8][4];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int64_t j = i + k * stride;
+      const uint64_t line = (uint64_t(mix32(uint64_t(j) ^ seed)) * n_lines) >> 32;
+      const uint4* at = src + line * uint64_t(line_u4);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[k][c] = (j < n_reads && c < chunk_u4) ? at[c] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc ^= v[k][c].x ^ v[k][c].y ^ v[k][c].z ^ v[k][c].w;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc ^= __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicXor(out, acc);
+}
+}  // namespace
+
+pa_status pa_gpu_gather_lines(pa_gpu_ctx* ctx, const void* d_src, size_t bytes, int32_t line_bytes, int32_t read_bytes,
+                              int64_t n_reads, uint64_t seed, uint32_t* d_xor) {
+  if (!ctx || !d_src || !d_xor || n_reads < 0) return PA_E_INVALID;
+  if (reinterpret_cast<uintptr_t>(d_src) & 15) return fail(ctx, PA_E_INVALID, "gather_lines: 16-byte aligned source");
+  if (line_bytes < 16 || line_bytes % 16 || read_bytes < 16 || read_bytes % 16 || read_bytes > 64 ||
+      read_bytes > line_bytes)
+    return fail(ctx, PA_E_INVALID, "gather_lines: lines and reads in 16-byte units, reads of 16 to 64 bytes");
+  const uint64_t n_lines = uint64_t(bytes) / uint64_t(line_bytes);
+  if (n_lines < 1 || n_lines > UINT32_MAX) return fail(ctx, PA_E_INVALID, "gather_lines: 1 to 2^32-1 lines");
+  if (!n_reads) return PA_OK;
+  int cus = 0;
+  PA_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  int64_t g = int64_t(cus) * 8;
+  const int64_t need = (n_reads + 255) / 256;
+  if (g > need) g = need;
+  launch_begin(ctx, "gather_lines");
+  hipLaunchKernelGGL(k_gather_lines, dim3(unsigned(g)), dim3(256), 0, ctx->stream, static_cast<const uint4*>(d_src),
+                     uint32_t(n_lines), line_bytes / 16, read_bytes / 16, n_reads, seed, reinterpret_cast<unsigned*>(d_xor));
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  return PA_OK;
+}
+
 pa_status pa_gpu_timing_enable(pa_gpu_ctx* ctx, int enable) {
   if (!ctx) return PA_E_INVALID;
   ctx->timing = enable != 0;
