@@ -560,7 +560,7 @@ pa_status   pa_gpu_pool_run_async(pa_gpu_tracer* tracer, const pa_gpu_field* con
  * launch it when needed.  Other work of the process on the same device may
  * share its hardware queue: stop the server before it.  particle_capacity
  * 64 .. 2^24 (68 B of pinned host memory and 20 B of device memory each);
- * waves_per_cu 0 = 8; idle_polls is reserved (0). */
+ * waves_per_cu 0 = 12; idle_polls is reserved (0). */
 pa_status   pa_gpu_serve_create(pa_gpu_tracer* tracer, const pa_gpu_field* const* fields, int32_t n_fields,
                                 size_t particle_capacity, int32_t request_slots, int32_t waves_per_cu,
                                 int32_t idle_polls, pa_gpu_server** out);

@@ -475,6 +475,51 @@ __device__ __forceinline__ int advance_rt(const TraceConfig& c, const FieldDesc&
   return finish_step<S, MODE, INTERP, STOP_RUNTIME>(c, f, last, v, next, rule);
 }
 
+// Whether a step loop over unrelated particles (the pool, the serving
+// kernel) keeps each lane's last cell record in LDS: 16-bit cell records
+// (three uint4) with one field evaluation per step.
+template <int S, int MODE>
+constexpr bool keeps_record() {
+  return layout_of(S) == PA_LAYOUT_CELLS && storage_of(S) != PA_STORAGE_FP32 && MODE != STEP_CLASSIC;
+}
+
+// advance_rt() for a lane that keeps its last cell record in LDS
+// (keep[0..2][threadIdx.x], cell index `kept`, 0xffffffff for none): a step
+// that stays in the kept cell reads the record from LDS, any other step
+// gathers it and keeps it.  Lanes that follow unrelated particles make a
+// vector-L1 miss of their own with every gather; most steps stay in their
+// cell and now make none.  The values are those of the gather.
+template <int S, int MODE, int INTERP, int BLOCK>
+__device__ __forceinline__ int advance_kept(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next,
+                                            int rule, uint32_t& kept, uint4 (*keep)[BLOCK]) {
+  float d[3], q[3], v[3];
+  if (!locate(f, last.x, last.y, last.z, d, q)) return 1;
+  int m[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) m[i] = static_cast<int>(floorf(q[i]));
+  // fewer than 2^31 records (checked when the layout is built), so no cell has index 0xffffffff
+  const uint32_t ci = uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
+  const bool hit = ci == kept;
+  Gather<S> g;
+  if (hit) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g.r[k] = keep[k][threadIdx.x];
+  } else {
+    const uint4* r = static_cast<const uint4*>(f.cells) + uint64_t(ci) * rec_size<S>();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g.r[k] = gload<uint4>(r, k);
+  }
+  float k[24];
+  gather_corners<S>(g, k);
+  lerp_planar<INTERP>(f, d, q, k, v);
+  if (!hit) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) keep[j][threadIdx.x] = g.r[j];
+    kept = ci;
+  }
+  return finish_step<S, MODE, INTERP, STOP_RUNTIME>(c, f, last, v, next, rule);
+}
+
 // Raw buffer stores whose descriptor (out_rsrc) covers byte offsets below
 // kDrop: a lane with nothing to store passes kDrop and the hardware drops
 // the store, so the whole wave issues it without a branch (the same wait
