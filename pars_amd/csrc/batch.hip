@@ -187,15 +187,7 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
         if (out != 0) {
           fin = true;
         } else {
-          if (V.slots) {
-            stage[staged][threadIdx.x] = next;
-            if (++staged == kStage) {
-#pragma unroll
-              for (int k = 0; k < kStage; ++k) V.slots[sfirst + k] = stage[k][threadIdx.x];
-              sfirst += kStage;
-              staged = 0;
-            }
-          }
+          if (V.slots) stage[staged++][threadIdx.x] = next;  // written below, with the wave's other lanes
           p = next;
           ++it;
           ++st;
@@ -223,6 +215,16 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
         }
         have = false;
       }
+    }
+    // The wave writes its staged vertices together, once a lane's stage is full: a lane's stores then
+    // go out every kStage iterations with the other lanes', not in a different iteration per lane.
+    // (Every lane then holds at most kStage - 1 at the top of an iteration.)
+    if (V.slots && __ballot(staged == kStage) != 0) {
+#pragma unroll
+      for (int k = 0; k < kStage; ++k)
+        if (k < staged) V.slots[sfirst + k] = stage[k][threadIdx.x];
+      sfirst += staged;
+      staged = 0;
     }
     // a lane whose entry has loaded takes it up: it steps from the next iteration
     if (pend) {
