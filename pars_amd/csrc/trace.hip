@@ -391,8 +391,10 @@ __global__ __launch_bounds__(kBlock) void k_flow_map(const TraceConfig cfg, int6
 // covers chunks 64 i .. 64 i + 63 of the wave's records in order (lane l:
 // chunk (64 i + l) % U of record (64 i + l) / U), so it touches about 64 / U
 // lines, and goes straight to LDS (no staging registers); every lane then
-// reads its own record from LDS.  Lanes whose record is not needed load a
-// chunk of a needed one (no extra lines).
+// reads its own record from LDS.  Only the chunks of records that are wanted
+// load (the others' lanes are masked off): a lane whose record is not
+// wanted keeps what its LDS slot holds, e.g. the record of the cell it was
+// in at the last step.
 template <int S>
 struct CoopRecords {
   uint64_t addr[64];
@@ -410,14 +412,14 @@ __device__ __forceinline__ void coop_gather(CoopRecords<S>& sh, bool want, const
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const uint64_t any = sh.addr[__builtin_ctzll(W)];
   auto* dst = (__attribute__((address_space(3))) uint4*)(&sh.rec[0][0]);
 #pragma unroll
   for (int i = 0; i < U; ++i) {
     const int g = i * 64 + lane, src = g / U, c = g % U;
-    const uint64_t a = ((W >> src) & 1) ? sh.addr[src] : any;
-    __builtin_amdgcn_global_load_lds(reinterpret_cast<const __attribute__((address_space(1))) void*>(a + 16u * c),
-                                     dst + i * 64, 16, 0, 0);
+    if ((W >> src) & 1)
+      __builtin_amdgcn_global_load_lds(
+          reinterpret_cast<const __attribute__((address_space(1))) void*>(sh.addr[src] + 16u * c), dst + i * 64, 16,
+          0, 0);
   }
   // Loads into LDS count in vmcnt and no fence waits for them: wait here
   // (vmcnt 0; expcnt and lgkmcnt left at their maxima) before any lane
@@ -447,6 +449,8 @@ __global__ __launch_bounds__(kBlock) void k_flow_map_cells(const TraceConfig cfg
   float4 pos = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   int R = R0, it = 1;
   int state = 3;  // advect_one's states: 1 inactive (written), 2 hand-off, 3 never stepped; 0 running
+  // the cell whose record the lane's LDS slot holds (fewer than 2^31 records: no cell is 0xffffffff)
+  uint32_t kept = 0xffffffffu;
   if (i < n) {
     k = int(i % sd2);
     j = int((i / sd2) % sd1);
@@ -460,16 +464,21 @@ __global__ __launch_bounds__(kBlock) void k_flow_map_cells(const TraceConfig cfg
     float d[3], q[3];
     bool inside = false;
     const uint4* mine = nullptr;
+    bool need = false;  // inside a cell other than the kept one
     if (run) {
       inside = locate(f, pos.x, pos.y, pos.z, d, q);
       if (inside) {
         int m[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) m[a] = static_cast<int>(floorf(q[a]));
+        const uint32_t ci =
+            uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
+        need = ci != kept;
+        kept = ci;
         mine = cell_record<S>(f, m);
       }
     }
-    coop_gather<S>(sh[w], inside, mine, lane);
+    coop_gather<S>(sh[w], need, mine, lane);
     if (run) {
       float4 next;
       int a = 1;
