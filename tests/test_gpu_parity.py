@@ -641,6 +641,41 @@ def test_pool_request_ids_checked(ctx, orc):
     assert r["position"][~on1].tobytes() == plain.results()["position"][~on1].tobytes()
 
 
+@pytest.mark.parametrize("n_requests", [1, 7, 2048, 2049, 5000])
+@pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
+def test_pool_request_counts(ctx, orc, n_requests, layout):
+    """request_done holds, per request id, the number of its entries that
+    finished: counted per workgroup in LDS and added once (up to 2,048
+    requests) or added per particle (more), exact either way, over bounded
+    passes and resident-wave counts that spread a request over many
+    workgroups."""
+    fields, fo = _pool_fields(ctx, orc, pa.BF16, layout)
+    t = pa.particle_tracer(ctx)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.01)
+    parts = _pool_particles(orc, fo, 3000, 11)
+    rng = np.random.default_rng(n_requests)
+    parts["original_rank"] = rng.integers(0, n_requests, len(parts))
+    if n_requests > 3:  # a hot request: a third of the entries
+        parts["original_rank"][rng.random(len(parts)) < 0.33] = n_requests - 1
+    n = len(parts)
+    want = np.bincount(parts["original_rank"], minlength=n_requests)
+    dev = pa.Particles.from_host(ctx, parts)
+    pool = pa.ParticlePool(ctx, n, requests=n_requests)
+    for waves, budget in ((0, 0), (4, 37)):
+        pool.request_done.upload(np.zeros(n_requests, np.int32))
+        pool.init(dev, 2)
+        passes = 0
+        while True:
+            pool.run(t, fields, max_iterations=budget, waves_per_cu=waves)
+            passes += 1
+            done = pool.request_done.download(np.int32, n_requests)
+            if (pool.status.download(np.int32, n) == pa.POOL_DONE).all():
+                break
+            assert budget and passes < 20 and (done <= want).all()
+        assert np.array_equal(done, want), (waves, budget)
+
+
 def test_serving_alone_equals_mix(ctx, orc):
     """Continuous batching: requests that arrive while others run share the
     pool passes, and every request's endpoints equal its result alone."""
