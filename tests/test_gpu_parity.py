@@ -587,6 +587,39 @@ def test_pool_batched_parity(ctx, orc, storage, layout):
     assert pool3.results()["position"].tobytes() == res["position"][sel].tobytes()
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 200])
+@pytest.mark.parametrize("waves", [0, 1, 64])
+def test_pool_small_and_oversubscribed(ctx, orc, n, waves):
+    """Pools smaller than, equal to and just past one wave's 64-entry claim,
+    on the default launch, one wave per CU and more waves than stay
+    resident (late workgroups find the counter past the end): every entry
+    ends with the oracle's endpoint and step count, and each request's
+    counter counts its entries once."""
+    fields, fo = _pool_fields(ctx, orc, pa.BF16, pa.LAYOUT_CELLS)
+    parts = _pool_particles(orc, fo, 100, 17)[:n]
+    t = pa.particle_tracer(ctx)
+    t.set_integrator("runge_kutta_4")
+    t.set_step_size(0.01)
+    pool = pa.ParticlePool(ctx, n, requests=2)
+    pool.init(pa.Particles.from_host(ctx, parts), 2)
+    total = pool.run(t, fields, waves_per_cu=waves)
+    res = pool.results()
+    assert (res["status"] == pa.POOL_DONE).all()
+    want_pos = np.zeros((n, 4), np.float32)
+    want_steps = np.zeros(n, np.int32)
+    for k in range(2):
+        sel = np.nonzero(parts["vector_field_index"] == k)[0]
+        if len(sel) == 0:
+            continue
+        q = parts[sel].copy()
+        q["vector_field_index"] = -1
+        want_pos[sel], want_steps[sel] = orc.endpoints(q, fo[k], dt=0.01, integrator=orc.RK4)
+    assert res["position"].tobytes() == want_pos.tobytes()
+    assert np.array_equal(res["steps"], want_steps) and total == int(want_steps.sum())
+    assert np.array_equal(pool.request_done.download(np.int32, 2),
+                          np.bincount(parts["original_rank"], minlength=2))
+
+
 def test_pool_rejects_bad_field_index(ctx, orc):
     fields, fo = _pool_fields(ctx, orc, pa.FP32, pa.LAYOUT_NODES)
     parts = _pool_particles(orc, fo, 10, 4)
