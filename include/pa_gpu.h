@@ -494,7 +494,8 @@ typedef struct pa_pool {
   int32_t* status;       /* PA_POOL_* */
   int32_t* field;        /* field index */
   int32_t* request;      /* optional: request id (the particle's original_rank) */
-  int32_t* request_done; /* optional: finished particles per request id (zeroed by the caller) */
+  int32_t* request_done; /* optional: finished particles per request id (zeroed by the caller; a
+                          * run adds its counts as its workgroups end, all of them by its end) */
   int32_t  n_requests;   /* entries of request_done; request ids are in [0, n_requests) */
 } pa_pool;
 /* Fill pool entries [0, n) from particles; entries whose field index is not
