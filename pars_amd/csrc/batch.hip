@@ -79,7 +79,8 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_init(const pa_particle* __r
 // Entries go to lanes in no fixed order; a particle's result does not depend
 // on it.
 template <int S, int MODE, int INTERP>
-__global__ __launch_bounds__(kPoolBlock) __attribute__((amdgpu_waves_per_eu(keeps_record<S, MODE>() ? 5 : 1)))
+__global__ __launch_bounds__(kPoolBlock)
+__attribute__((amdgpu_waves_per_eu(keeps_record<S, MODE>() && storage_of(S) != PA_STORAGE_FP32 ? 5 : 1)))
 void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
                                                          int32_t* __restrict__ it_a, const int32_t* __restrict__ rem,
                                                          int32_t* __restrict__ steps_a,
@@ -102,9 +103,12 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
     for (int k = threadIdx.x; k < kWords; k += kPoolBlock) dst[k] = src[k];
     __syncthreads();
   }
-  // the workgroup's request counters (dynamic LDS, sized n_requests when counted here)
-  extern __shared__ int32_t req_count[];
+  // Dynamic LDS: the workgroup's request counters (n_requests ints, when counted here), then, for vertex
+  // output, the staged vertices (kStage x kPoolBlock vec4); sized at launch (pool_lds_bytes).
+  extern __shared__ uint4 dyn_lds[];
+  int32_t* req_count = reinterpret_cast<int32_t*>(dyn_lds);
   const bool lds_count = request && request_done && n_requests <= kLdsRequests;
+  auto stage = reinterpret_cast<float4(*)[kPoolBlock]>(dyn_lds + (lds_count ? (n_requests + 3) / 4 : 0));
   if (lds_count) {
     for (int k = threadIdx.x; k < n_requests; k += kPoolBlock) req_count[k] = 0;
     __syncthreads();
@@ -118,11 +122,10 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
   unsigned long long cur = 0, end = 0, ahead = 0;
   bool claimed = true;
   if (lane == 0) ahead = atomicAdd(work, csize);
-  // Vertex output is staged per lane in LDS and written kStage slots at a time: a global store counts
-  // in the same in-order wait as the next gather, so one per step would lengthen every step.
-  __shared__ float4 stage[kStage][kPoolBlock];
+  // Vertex output is staged per lane in LDS (stage, above) and written kStage slots at a time: a global
+  // store counts in the same in-order wait as the next gather, so one per step would lengthen every step.
   constexpr bool kKeep = keeps_record<S, MODE>();
-  __shared__ uint4 keep[kKeep ? 3 : 1][kPoolBlock];  // the lane's last cell record (kKeep)
+  __shared__ uint4 keep[kKeep ? rec_used<S>() : 1][kPoolBlock];  // the lane's last cell record (kKeep)
   uint32_t kept = 0xffffffffu;                        // its cell index; none
   int staged = 0;       // vertices waiting in LDS
   int64_t sfirst = 0;   // slot of the first of them
@@ -370,9 +373,11 @@ pa_status pool_launch(pa_gpu_tracer* t, const pa_gpu_field* const* fields, int32
   hipLaunchKernelGGL(k_pool_zero, dim3(1), dim3(64), 0, ctx->stream, counters);
   const int budget = max_iterations > 0 ? max_iterations : INT32_MAX;
   hipError_t occ = hipSuccess;
-  // the request counters in LDS (k_pool_run counts them there up to kLdsRequests requests)
+  // k_pool_run's dynamic LDS: the request counters (counted there up to kLdsRequests requests), 16-byte
+  // aligned, then the vertex stage of vertex output
   const bool lds_count = pool->request && pool->request_done && pool->n_requests <= kLdsRequests;
-  const size_t lds_bytes = lds_count ? size_t(pool->n_requests) * sizeof(int32_t) : 0;
+  const size_t lds_bytes = (lds_count ? (size_t(pool->n_requests) + 3) / 4 * 16 : 0) +
+                           (V.slots ? size_t(kStage) * kPoolBlock * sizeof(float4) : 0);
   launch_begin(ctx, "pool_run");
   dispatch(cfg.f[0].storage, mode_of_integrator(t->integrator), cfg.k.interp, [&](auto S, auto M, auto I) {
     const auto kernel = k_pool_run<decltype(S)::value, decltype(M)::value, decltype(I)::value>;

@@ -476,15 +476,16 @@ __device__ __forceinline__ int advance_rt(const TraceConfig& c, const FieldDesc&
 }
 
 // Whether a step loop over unrelated particles (the pool, the serving
-// kernel) keeps each lane's last cell record in LDS: 16-bit cell records
-// (three uint4) with one field evaluation per step.
+// kernel) keeps each lane's last cell record in LDS (rec_used<S>() uint4 a
+// lane): cell records with one field evaluation per step.
 template <int S, int MODE>
 constexpr bool keeps_record() {
-  return layout_of(S) == PA_LAYOUT_CELLS && storage_of(S) != PA_STORAGE_FP32 && MODE != STEP_CLASSIC;
+  return layout_of(S) == PA_LAYOUT_CELLS && MODE != STEP_CLASSIC;
 }
 
 // advance_rt() for a lane that keeps its last cell record in LDS
-// (keep[0..2][threadIdx.x], cell index `kept`, 0xffffffff for none): a step
+// (keep[0..U-1][threadIdx.x], U = rec_used<S>(); cell index `kept`,
+// 0xffffffff for none): a step
 // that stays in the kept cell reads the record from LDS, any other step
 // gathers it and keeps it.  Lanes that follow unrelated particles make a
 // vector-L1 miss of their own with every gather; most steps stay in their
@@ -501,20 +502,21 @@ __device__ __forceinline__ int advance_kept(const TraceConfig& c, const FieldDes
   const uint32_t ci = uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
   const bool hit = ci == kept;
   Gather<S> g;
+  constexpr int U = rec_used<S>();
   if (hit) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) g.r[k] = keep[k][threadIdx.x];
+    for (int k = 0; k < U; ++k) g.r[k] = keep[k][threadIdx.x];
   } else {
     const uint4* r = static_cast<const uint4*>(f.cells) + uint64_t(ci) * rec_size<S>();
 #pragma unroll
-    for (int k = 0; k < 3; ++k) g.r[k] = gload<uint4>(r, k);
+    for (int k = 0; k < U; ++k) g.r[k] = gload<uint4>(r, k);
   }
   float k[24];
   gather_corners<S>(g, k);
   lerp_planar<INTERP>(f, d, q, k, v);
   if (!hit) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) keep[j][threadIdx.x] = g.r[j];
+    for (int j = 0; j < U; ++j) keep[j][threadIdx.x] = g.r[j];
     kept = ci;
   }
   return finish_step<S, MODE, INTERP, STOP_RUNTIME>(c, f, last, v, next, rule);

@@ -536,7 +536,8 @@ def _pool_particles(orc, fields_o, n_per, seed):
     return allp[rng.permutation(len(allp))]
 
 
-@pytest.mark.parametrize("storage,layout", [(pa.FP32, pa.LAYOUT_NODES), (pa.BF16, pa.LAYOUT_CELLS)])
+@pytest.mark.parametrize("storage,layout", [(pa.FP32, pa.LAYOUT_NODES), (pa.BF16, pa.LAYOUT_CELLS),
+                                            (pa.FP32, pa.LAYOUT_CELLS), (pa.FP16, pa.LAYOUT_CELLS)])
 def test_pool_batched_parity(ctx, orc, storage, layout):
     """Batched seed sets over two fields in one persistent kernel: every
     endpoint and step count equals the oracle's one-block trace; bounded
