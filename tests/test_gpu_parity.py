@@ -758,14 +758,17 @@ def test_device_eigen_solver_on_lapack_fixtures(ctx):
     assert np.all(np.diff(got, axis=1) >= -1e-12 * scale[:, None])  # ascending
 
 
+@pytest.mark.parametrize("storage,layout", [(pa.BF16, pa.LAYOUT_NODES), (pa.BF16, pa.LAYOUT_CELLS),
+                                            (pa.FP32, pa.LAYOUT_CELLS)])
 @pytest.mark.parametrize("capacity", [60000, 4500])
-def test_resident_serving_alone_equals_mix(ctx, orc, capacity):
+def test_resident_serving_alone_equals_mix(ctx, orc, capacity, storage, layout):
     """The device-resident serving loop: requests published while others run
     overlap on the device, a small ring admits them as space frees up, and
-    every request's endpoints and steps equal its result alone."""
+    every request's endpoints and steps equal its result alone and the
+    oracle's (node bricks, and cell records kept per lane between steps)."""
     from pars_amd import serve as S
     grid = 64
-    field = S.make_field(ctx, pa.BF16, grid=grid, n_modes=8)
+    field = S.make_field(ctx, storage, grid=grid, n_modes=8).set_layout(layout)
     reqs = S.make_requests(14, rate=20000.0, grid=grid, steps=120, size_range=(64, 4000))
     server = S.ResidentServer(ctx, [field], capacity=capacity, request_slots=8, waves_per_cu=4)
     steps, wall = server.serve(reqs)
