@@ -25,7 +25,8 @@ def _written(idx, n):
     return w
 
 
-@pytest.mark.parametrize("storage,layout", [(pa.BF16, pa.LAYOUT_CELLS), (pa.FP32, pa.LAYOUT_NODES)])
+@pytest.mark.parametrize("storage,layout", [(pa.BF16, pa.LAYOUT_CELLS), (pa.FP32, pa.LAYOUT_NODES),
+                                            (pa.FP32, pa.LAYOUT_CELLS)])
 def test_batch_vertices_equal_reference_rounds(ctx, orc, storage, layout):
     wl = B.make_workload(grids=(16, 24, 20), n_sets=6, n_modes=8, size_range=(60, 700), steps_range=(20, 150))
     fields = B.build_fields(ctx, wl, storage, layout)
