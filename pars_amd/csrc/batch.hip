@@ -80,7 +80,8 @@ __global__ __launch_bounds__(kPoolBlock) void k_pool_init(const pa_particle* __r
 // on it.
 template <int S, int MODE, int INTERP>
 __global__ __launch_bounds__(kPoolBlock)
-__attribute__((amdgpu_waves_per_eu(keeps_record<S, MODE>() && storage_of(S) != PA_STORAGE_FP32 ? 5 : 1)))
+__attribute__((amdgpu_waves_per_eu(keeps_record<S, MODE>() && layout_of(S) == PA_LAYOUT_CELLS &&
+                                           storage_of(S) != PA_STORAGE_FP32 ? 5 : 1)))
 void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
                                                          int32_t* __restrict__ it_a, const int32_t* __restrict__ rem,
                                                          int32_t* __restrict__ steps_a,
@@ -125,7 +126,7 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
   // Vertex output is staged per lane in LDS (stage, above) and written kStage slots at a time: a global
   // store counts in the same in-order wait as the next gather, so one per step would lengthen every step.
   constexpr bool kKeep = keeps_record<S, MODE>();
-  __shared__ uint4 keep[kKeep ? rec_used<S>() : 1][kPoolBlock];  // the lane's last cell record (kKeep)
+  __shared__ uint4 keep[kKeep ? keep_words<S>() : 1][kPoolBlock];  // the lane's last cell's corners (kKeep)
   uint32_t kept = 0xffffffffu;                        // its cell index; none
   int staged = 0;       // vertices waiting in LDS
   int64_t sfirst = 0;   // slot of the first of them

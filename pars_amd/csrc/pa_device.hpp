@@ -476,20 +476,28 @@ __device__ __forceinline__ int advance_rt(const TraceConfig& c, const FieldDesc&
 }
 
 // Whether a step loop over unrelated particles (the pool, the serving
-// kernel) keeps each lane's last cell record in LDS (rec_used<S>() uint4 a
-// lane): cell records with one field evaluation per step.
+// kernel) keeps each lane's last cell's corner data in LDS
+// (keep_words<S>() uint4 a lane), for the steppers with one field
+// evaluation per step.
 template <int S, int MODE>
 constexpr bool keeps_record() {
-  return layout_of(S) == PA_LAYOUT_CELLS && MODE != STEP_CLASSIC;
+  return MODE != STEP_CLASSIC;
+}
+// uint4 a lane keeps: a cell record's used part; FP32 nodes, the 8 corner
+// nodes; 16-bit nodes, the 8 corners as uint2, two to a uint4.
+template <int S>
+constexpr int keep_words() {
+  return layout_of(S) == PA_LAYOUT_CELLS ? rec_used<S>() : storage_of(S) == PA_STORAGE_FP32 ? 8 : 4;
 }
 
-// advance_rt() for a lane that keeps its last cell record in LDS
-// (keep[0..U-1][threadIdx.x], U = rec_used<S>(); cell index `kept`,
-// 0xffffffff for none): a step
-// that stays in the kept cell reads the record from LDS, any other step
-// gathers it and keeps it.  Lanes that follow unrelated particles make a
-// vector-L1 miss of their own with every gather; most steps stay in their
-// cell and now make none.  The values are those of the gather.
+// advance_rt() for a lane that keeps its last cell's corner data in LDS
+// (keep[0..keep_words<S>()-1][threadIdx.x]; `kept` names the cell: its
+// record's index, or its corner-000 node's; 0xffffffff for none, which no
+// cell has: fewer than 2^31 records and 2^32 nodes): a step that stays in
+// the kept cell reads the corners from LDS, any other step gathers them and
+// keeps them.  Lanes that follow unrelated particles make vector-L1 misses
+// of their own with every gather; most steps stay in their cell and now
+// make none.  The values are those of the gather.
 template <int S, int MODE, int INTERP, int BLOCK>
 __device__ __forceinline__ int advance_kept(const TraceConfig& c, const FieldDesc& f, float4 last, float4& next,
                                             int rule, uint32_t& kept, uint4 (*keep)[BLOCK]) {
@@ -498,25 +506,44 @@ __device__ __forceinline__ int advance_kept(const TraceConfig& c, const FieldDes
   int m[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) m[i] = static_cast<int>(floorf(q[i]));
-  // fewer than 2^31 records (checked when the layout is built), so no cell has index 0xffffffff
-  const uint32_t ci = uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) + uint32_t(m[2]);
+  constexpr bool kCells = layout_of(S) == PA_LAYOUT_CELLS;
+  const uint32_t ci = kCells ? uint32_t(m[0]) * uint32_t(f.cstride0) + uint32_t(m[1]) * uint32_t(f.cstride1) +
+                                   uint32_t(m[2])
+                             : uint32_t(m[0]) * uint32_t(f.stride0) + uint32_t(m[1]) * uint32_t(f.stride1) +
+                                   uint32_t(m[2]);
   const bool hit = ci == kept;
+  constexpr int W = keep_words<S>();
+  constexpr bool kPairs = !kCells && storage_of(S) != PA_STORAGE_FP32;  // two uint2 nodes to a uint4
   Gather<S> g;
-  constexpr int U = rec_used<S>();
   if (hit) {
 #pragma unroll
-    for (int k = 0; k < U; ++k) g.r[k] = keep[k][threadIdx.x];
-  } else {
+    for (int k = 0; k < W; ++k) {
+      const uint4 w = keep[k][threadIdx.x];
+      if constexpr (kPairs) {
+        g.r[2 * k] = make_uint4(w.x, w.y, 0u, 0u);
+        g.r[2 * k + 1] = make_uint4(w.z, w.w, 0u, 0u);
+      } else {
+        g.r[k] = w;
+      }
+    }
+  } else if constexpr (kCells) {
     const uint4* r = static_cast<const uint4*>(f.cells) + uint64_t(ci) * rec_size<S>();
 #pragma unroll
-    for (int k = 0; k < U; ++k) g.r[k] = gload<uint4>(r, k);
+    for (int k = 0; k < W; ++k) g.r[k] = gload<uint4>(r, k);
+  } else {
+    gather_issue<S>(f, m, g);
   }
   float k[24];
   gather_corners<S>(g, k);
   lerp_planar<INTERP>(f, d, q, k, v);
   if (!hit) {
 #pragma unroll
-    for (int j = 0; j < U; ++j) keep[j][threadIdx.x] = g.r[j];
+    for (int j = 0; j < W; ++j) {
+      if constexpr (kPairs)
+        keep[j][threadIdx.x] = make_uint4(g.r[2 * j].x, g.r[2 * j].y, g.r[2 * j + 1].x, g.r[2 * j + 1].y);
+      else
+        keep[j][threadIdx.x] = g.r[j];
+    }
     kept = ci;
   }
   return finish_step<S, MODE, INTERP, STOP_RUNTIME>(c, f, last, v, next, rule);

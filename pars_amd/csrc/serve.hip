@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kServeBlock) void k_serve(const ServeConfig cfg, co
   }
   __shared__ u32x4 staged[kServeBlock / 64][64 * kWords / 4];
   constexpr bool kKeep = keeps_record<S, MODE>();
-  __shared__ uint4 keep[kKeep ? rec_used<S>() : 1][kServeBlock];  // the lane's last cell record (kKeep)
+  __shared__ uint4 keep[kKeep ? keep_words<S>() : 1][kServeBlock];  // the lane's last cell's corners (kKeep)
   uint32_t kept = 0xffffffffu;                         // its cell index; none
   const int lane = threadIdx.x & 63;
   u32x4* mine = staged[threadIdx.x >> 6];
