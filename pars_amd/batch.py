@@ -129,10 +129,14 @@ class Tuning:
 
 
 def autotune(ctx: Context, wl: BatchWorkload, storage=BF16, layouts=(LAYOUT_NODES, LAYOUT_CELLS),
-             waves=(4, 8, 16, 32), sample_stride: int = 16, cache: Optional[str] = None) -> Tuning:
-    """Time every (layout, waves per CU) on every sample_stride-th seed of the
-    batch and return the fastest.  With `cache`, a previous result for the
-    same device and batch shape is reused."""
+             waves=(4, 8, 16, 0), sample_stride: int = 16, cache: Optional[str] = None) -> Tuning:
+    """Time every (layout, waves per CU; 0 = as many as stay resident) on
+    every sample_stride-th seed of the batch, best of two runs, and return
+    the fastest; a full-occupancy launch within 3% of it is preferred (the
+    sample's short runs understate what occupancy gives the whole batch:
+    16 waves per CU won on the sample at 54.3 against 53.8 G steps/s and ran
+    the batch in 33.8 ms against 31.6).  With `cache`, a previous result for
+    the same device and batch shape is reused."""
     key = "%s|%d|%d|%d" % (ctx.device_info()["name"].strip(), wl.n, len(wl.sets), storage)
     if cache and os.path.exists(cache):
         with open(cache) as fh:
@@ -151,17 +155,21 @@ def autotune(ctx: Context, wl: BatchWorkload, storage=BF16, layouts=(LAYOUT_NODE
         for wv in waves:
             pool.init(dev, len(fields))
             pool.run(t, fields, waves_per_cu=wv)  # warm
-            pool.init(dev, len(fields))
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            steps = pool.run(t, fields, waves_per_cu=wv)
-            dt = time.perf_counter() - t0
-            rate = steps / dt
-            table["%s/w%d" % ("cells" if lay == LAYOUT_CELLS else "nodes", wv)] = rate
+            rate = 0.0
+            for _ in range(2):
+                pool.init(dev, len(fields))
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                steps = pool.run(t, fields, waves_per_cu=wv)
+                rate = max(rate, steps / (time.perf_counter() - t0))
+            table["%s/%s" % ("cells" if lay == LAYOUT_CELLS else "nodes", "wmax" if wv == 0 else "w%d" % wv)] = rate
             if best is None or rate > best[2]:
                 best = (lay, wv, rate)
         for f in fields:
             f.release()
+    full = table.get("%s/wmax" % ("cells" if best[0] == LAYOUT_CELLS else "nodes"))
+    if full is not None and full >= 0.97 * best[2]:
+        best = (best[0], 0, full)
     tun = Tuning(best[0], best[1], best[2], table)
     if cache:
         known = {}

@@ -26,8 +26,9 @@ constexpr int kChunk = 64;  // pool entries a wave claims at a time
 // Request counters a workgroup keeps in LDS (dynamic, n_requests ints) and adds to request_done once, when it
 // exits: one add per particle on a request's single counter serialises behind the others and holds up the
 // wave's next gather wait (batched workload: 68.8 ms with them, 31 ms without).  More requests than this
-// are counted with one add per particle.
-constexpr int kLdsRequests = 2048;
+// are counted with one add per particle.  4,096 counters (16 KB) leave 16-bit cell records at 5 waves per
+// SIMD.
+constexpr int kLdsRequests = 4096;
 
 struct PoolConfig {
   TraceConfig k;                // integrator constants, stop rule, interpolation

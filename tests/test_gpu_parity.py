@@ -675,11 +675,11 @@ def test_pool_request_ids_checked(ctx, orc):
     assert r["position"][~on1].tobytes() == plain.results()["position"][~on1].tobytes()
 
 
-@pytest.mark.parametrize("n_requests", [1, 7, 2048, 2049, 5000])
+@pytest.mark.parametrize("n_requests", [1, 7, 4096, 4097, 6000])
 @pytest.mark.parametrize("layout", [pa.LAYOUT_NODES, pa.LAYOUT_CELLS])
 def test_pool_request_counts(ctx, orc, n_requests, layout):
     """request_done holds, per request id, the number of its entries that
-    finished: counted per workgroup in LDS and added once (up to 2,048
+    finished: counted per workgroup in LDS and added once (up to 4,096
     requests) or added per particle (more), exact either way, over bounded
     passes and resident-wave counts that spread a request over many
     workgroups."""
