@@ -289,17 +289,35 @@ __global__ __launch_bounds__(kBlock) void k_interpolate(const FieldDesc f, const
 }
 
 // ----------------------------------------------------------------- advect
+// Whether advect_one keeps each lane's last cell's corner data in LDS
+// (advance_kept): for cell records and FP32 nodes.  A lane reads memory only
+// when its particle enters another cell (the FTLE flow map on ABC 256^3 FP32
+// nodes: 69.6 -> 50.5 ms).  16-bit nodes gather from the lines that their
+// neighbouring lanes read as well, and the kept copy costs registers and
+// occupancy (56 -> 91 VGPRs): the same flow map on BF16 nodes went
+// 57.3 -> 61.7 ms, so they gather every step.
+template <int S, int MODE>
+constexpr bool lane_keeps() {
+  return keeps_record<S, MODE>() && (layout_of(S) == PA_LAYOUT_CELLS || storage_of(S) == PA_STORAGE_FP32);
+}
+
 // particle_advector.cpp:61-141 in place.  state 1 inactive, 2 hand-off to
-// slot, 3 never stepped (R <= 1).
+// slot, 3 never stepped (R <= 1).  keep: the block's
+// [lane_keeps ? keep_words : 1][kBlock] LDS array.
 template <int S, int MODE, int INTERP>
 __device__ __forceinline__ int advect_one(const TraceConfig& cfg, const FieldDesc& f, float4& pos, int& R, int& slot,
-                                          unsigned long long& steps, int stop_rule) {
+                                          unsigned long long& steps, int stop_rule, uint4 (*keep)[kBlock]) {
   int state = 3;
   slot = -1;
+  uint32_t kept = 0xffffffffu;  // the cell whose corners keep holds: none yet
   for (int it = 1; it < R; ++it) {
     float4 next;
-    const int a = stop_rule == PA_STOP_NORM ? advance<S, MODE, INTERP, PA_STOP_NORM>(cfg, f, pos, next)
-                                            : advance<S, MODE, INTERP, PA_STOP_ISZERO>(cfg, f, pos, next);
+    int a;
+    if constexpr (lane_keeps<S, MODE>())
+      a = advance_kept<S, MODE, INTERP, kBlock>(cfg, f, pos, next, stop_rule, kept, keep);
+    else
+      a = stop_rule == PA_STOP_NORM ? advance<S, MODE, INTERP, PA_STOP_NORM>(cfg, f, pos, next)
+                                    : advance<S, MODE, INTERP, PA_STOP_ISZERO>(cfg, f, pos, next);
     if (a == 1) {  // :76-100
       R -= it;
       const int k = face_rule(f, pos, cfg.local_mask);
@@ -330,6 +348,8 @@ template <int S, int MODE, int INTERP>
 __global__ __launch_bounds__(kBlock) void k_advect(const TraceConfig cfg, pa_particle* __restrict__ P, int64_t n,
                                                    int32_t* __restrict__ state, int32_t* __restrict__ slot,
                                                    unsigned long long* __restrict__ steps_out, int stop_rule) {
+  // each lane's last cell's corners (the particles are unrelated after a few steps)
+  __shared__ uint4 keep[lane_keeps<S, MODE>() ? keep_words<S>() : 1][kBlock];  // each lane's last cell's corners
   const int64_t pi = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   unsigned long long steps = 0;
   if (pi < n) {
@@ -337,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_advect(const TraceConfig cfg, pa_par
     float4 pos = make_float4(p.position[0], p.position[1], p.position[2], p.position[3]);
     int R = p.remaining_iterations;
     int sl;
-    const int st = advect_one<S, MODE, INTERP>(cfg, cfg.field[0], pos, R, sl, steps, stop_rule);
+    const int st = advect_one<S, MODE, INTERP>(cfg, cfg.field[0], pos, R, sl, steps, stop_rule, keep);
     p.position[0] = pos.x;
     p.position[1] = pos.y;
     p.position[2] = pos.z;
@@ -368,13 +388,14 @@ __global__ __launch_bounds__(kBlock) void k_flow_map(const TraceConfig cfg, int6
                                                      float o1, float o2, float s0, float s1, float s2, int32_t R0,
                                                      float* __restrict__ fm, int fd0, int fd1, int fd2,
                                                      unsigned long long* __restrict__ steps_out, int stop_rule) {
+  __shared__ uint4 keep[lane_keeps<S, MODE>() ? keep_words<S>() : 1][kBlock];  // each lane's last cell's corners
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   unsigned long long steps = 0;
   if (i < n) {
     const int k = int(i % sd2), j = int((i / sd2) % sd1), x = int(i / (int64_t(sd1) * sd2));
     float4 pos = make_float4(o0 + s0 * float(x), o1 + s1 * float(j), o2 + s2 * float(k), 0.0f);
     int R = R0, sl;
-    const int st = advect_one<S, MODE, INTERP>(cfg, cfg.field[0], pos, R, sl, steps, stop_rule);
+    const int st = advect_one<S, MODE, INTERP>(cfg, cfg.field[0], pos, R, sl, steps, stop_rule, keep);
     if (st == 1 && x < fd0 && j < fd1 && k < fd2) {
       const int64_t at = (int64_t(x) * fd1 + j) * fd2 + k;
       fm[3 * at] = pos.x;
