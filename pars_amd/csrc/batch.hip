@@ -224,10 +224,21 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
     // The wave writes its staged vertices together, once a lane's stage is full: a lane's stores then
     // go out every kStage iterations with the other lanes', not in a different iteration per lane.
     // (Every lane then holds at most kStage - 1 at the top of an iteration.)
+    // Four lanes write one lane's run (its kStage slots are consecutive), so that a store instruction
+    // covers 16 runs of up to 64 B rather than 64 scattered 16-B pieces.
     if (V.slots && __ballot(staged == kStage) != 0) {
+      static_assert(kStage == 4, "a run is written by 4 lanes");
+      const int k = lane & 3;
+      const int wave_base = threadIdx.x & ~63;
+      __builtin_amdgcn_wave_barrier();  // the wave's stage writes above come first
 #pragma unroll
-      for (int k = 0; k < kStage; ++k)
-        if (k < staged) V.slots[sfirst + k] = stage[k][threadIdx.x];
+      for (int g = 0; g < 4; ++g) {
+        const int src = g * 16 + (lane >> 2);
+        const int n_src = __shfl(staged, src);
+        const long long f_src = __shfl(static_cast<long long>(sfirst), src);
+        const float4 v = stage[k][wave_base + src];
+        if (k < n_src) V.slots[f_src + k] = v;
+      }
       sfirst += staged;
       staged = 0;
     }
