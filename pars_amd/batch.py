@@ -266,8 +266,8 @@ def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField
     advances every admitted particle by at most pass_iterations per pass (0:
     to its end; pa_gpu_pool_run_vertices: each particle writes its one-block
     reference slots into its set's part of the arena).  Bounded passes hand
-    sets out sooner (on the batched workload: passes of 400 iterations 83
-    ms of pool time, to the end 77 ms).  After each pass, a set whose particles have all ended is finished on the device
+    sets out sooner (on the batched workload: passes of 400 iterations 75
+    ms of pool time, to the end 63 ms).  After each pass, a set whose particles have all ended is finished on the device
     in chunks of at most chunk_vertices slots, as the reference chunks its
     curves (chunk = chunk_vertices // R seeds): counted prune, index, colour;
     each chunk goes to consumer(CurveChunk), then the set's arena range takes

@@ -116,6 +116,7 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;
+  __shared__ uint8_t order[kPoolBlock];  // lanes whose staged run goes out, in lane order (a wave each 64)
   const uint64_t below = (uint64_t(1) << lane) - 1;
   const unsigned long long un = static_cast<unsigned long long>(n);
   const unsigned long long csize = static_cast<unsigned long long>(chunk);
@@ -221,26 +222,35 @@ void k_pool_run(const PoolConfig cfg, float4* __restrict__ pos,
         have = false;
       }
     }
-    // The wave writes its staged vertices together, once a lane's stage is full: a lane's stores then
-    // go out every kStage iterations with the other lanes', not in a different iteration per lane.
-    // (Every lane then holds at most kStage - 1 at the top of an iteration.)
-    // Four lanes write one lane's run (its kStage slots are consecutive), so that a store instruction
-    // covers 16 runs of up to 64 B rather than 64 scattered 16-B pieces.
-    if (V.slots && __ballot(staged == kStage) != 0) {
-      static_assert(kStage == 4, "a run is written by 4 lanes");
-      const int k = lane & 3;
-      const int wave_base = threadIdx.x & ~63;
-      __builtin_amdgcn_wave_barrier();  // the wave's stage writes above come first
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int src = g * 16 + (lane >> 2);
-        const int n_src = __shfl(staged, src);
-        const long long f_src = __shfl(static_cast<long long>(sfirst), src);
-        const float4 v = stage[k][wave_base + src];
-        if (k < n_src) V.slots[f_src + k] = v;
+    // Staged vertices go out in runs that end on a 64-B boundary of the vertex buffer (4 slots): a lane
+    // writes its run once the run reaches one, so the lines see whole 64-B writes rather than pieces
+    // (only a particle's first and last runs are shorter).  The wave writes the runs of all such lanes
+    // together, four lanes a run (a store instruction covers 16 runs of up to 64 B, not 64 scattered
+    // 16-B pieces).  A lane adds at most one vertex an iteration and a run ends at the latest at the
+    // fourth slot, so a lane holds at most kStage.
+    if (V.slots) {
+      static_assert(kStage == 4, "a run is written by 4 lanes and ends on a 4-slot boundary");
+      const bool full = staged > 0 && int(sfirst & 3) + staged >= kStage;
+      const uint64_t F = __ballot(full);
+      if (F != 0) {
+        const int wave_base = threadIdx.x & ~63;
+        if (full) order[wave_base + __popcll(F & below)] = static_cast<uint8_t>(lane);
+        __builtin_amdgcn_wave_barrier();  // the wave's stage and order writes above come first
+        const int nf = __popcll(F);
+        const int k = lane & 3;
+        for (int g = 0; g < nf; g += 16) {
+          const int rr = g + (lane >> 2);
+          const int src = rr < nf ? int(order[wave_base + rr]) : lane;
+          const int n_src = __shfl(staged, src);
+          const long long f_src = __shfl(static_cast<long long>(sfirst), src);
+          const float4 v = stage[k][wave_base + src];
+          if (rr < nf && k < n_src) V.slots[f_src + k] = v;
+        }
+        if (full) {
+          sfirst += staged;
+          staged = 0;
+        }
       }
-      sfirst += staged;
-      staged = 0;
     }
     // a lane whose entry has loaded takes it up: it steps from the next iteration
     if (pend) {
