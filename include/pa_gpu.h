@@ -470,6 +470,22 @@ pa_status   pa_gpu_prune_counted_step_major(pa_gpu_ctx* ctx, const float* d_slot
 pa_status   pa_gpu_prune_counted_group8(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
                                         const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,
                                         size_t* n_out, int64_t* d_offsets);
+/* pa_gpu_prune_counted, pa_gpu_index_generate and pa_gpu_color_generate of
+ * one chunk in one pass over the slots (pipeline.cpp:186-260 runs
+ * particle_tracer's prune, index_generator.cpp:10-24 and
+ * color_generator.cpp:16-98 in turn on each chunk): the same vertices,
+ * indices and colours at the positions color_generate writes.  Vertices at
+ * most capacity and below 2^31 (int32 indices); d_offsets optional (n + 1
+ * int64).  Curves of the shape the tracers write (count >= 2, one
+ * termination vertex, last) are indexed and coloured in the same pass, and
+ * the other colour positions keep what d_colors held; a chunk with any
+ * other curve is indexed and coloured by the general kernels afterwards,
+ * and there the positions that color_generate does not write are
+ * undefined. */
+pa_status   pa_gpu_finish_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t stride_slots,
+                                  const int32_t* d_vertex_counts, size_t n, float* d_vertices, size_t capacity,
+                                  size_t* n_vertices, int64_t* d_offsets, int32_t* d_indices, size_t* n_indices,
+                                  pa_color_mode mode, float parameter, float* d_colors);
 /* Counted prune of chunked slots (PA_SLOTS_CHUNKED) into the pruned layout. */
 pa_status   pa_gpu_prune_counted_chunked(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
                                          const int32_t* d_vertex_counts, size_t n, float* d_out, size_t capacity,

@@ -269,7 +269,8 @@ def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField
     sets out sooner (on the batched workload: passes of 400 iterations 75
     ms of pool time, to the end 63 ms).  After each pass, a set whose particles have all ended is finished on the device
     in chunks of at most chunk_vertices slots, as the reference chunks its
-    curves (chunk = chunk_vertices // R seeds): counted prune, index, colour;
+    curves (chunk = chunk_vertices // R seeds): counted prune, index and colour
+    in one pass over the slots (pa_gpu_finish_counted);
     each chunk goes to consumer(CurveChunk), then the set's arena range takes
     the next sets.  Returns totals (steps, vertices, indices, sets, passes,
     seconds)."""
@@ -334,15 +335,12 @@ def run_batch_vertices(ctx: Context, wl: BatchWorkload, fields: List[VectorField
         for k, j0 in enumerate(range(0, int(sizes[s]), per)):
             j1 = min(int(sizes[s]), j0 + per)
             m, ni = C.c_size_t(), C.c_size_t()
-            ctx.check(L.pa_gpu_prune_counted(ctx.handle, C.c_void_p(arena.at((off + j0 * R) * 16)), C.c_size_t(R),
-                                             C.c_void_p(counts.at((a + j0) * 4)), C.c_size_t(j1 - j0),
-                                             C.c_void_p(vbuf.ptr), C.c_size_t(cap), C.byref(m), C.c_void_p(offs.ptr)),
-                      "prune_counted")
-            ctx.check(L.pa_gpu_index_generate(ctx.handle, C.c_void_p(vbuf.ptr), m, C.c_void_p(ibuf.ptr),
-                                              C.byref(ni)), "index")
-            ctx.check(L.pa_gpu_color_generate(ctx.handle, C.c_void_p(vbuf.ptr), m, C.c_void_p(ibuf.ptr), ni,
+            ctx.check(L.pa_gpu_finish_counted(ctx.handle, C.c_void_p(arena.at((off + j0 * R) * 16)), C.c_size_t(R),
+                                              C.c_void_p(counts.at((a + j0) * 4)), C.c_size_t(j1 - j0),
+                                              C.c_void_p(vbuf.ptr), C.c_size_t(cap), C.byref(m),
+                                              C.c_void_p(offs.ptr), C.c_void_p(ibuf.ptr), C.byref(ni),
                                               C.c_int(color_mode), C.c_float(color_parameter),
-                                              C.c_void_p(cbuf.ptr)), "color")
+                                              C.c_void_p(cbuf.ptr)), "finish_counted")
             tot["vertices"] += m.value
             tot["indices"] += ni.value
             tot["chunks"] += 1

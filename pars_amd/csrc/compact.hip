@@ -3,6 +3,7 @@
 // LDS prefix sums), the counted prune of a traced round, and the grouping
 // of hand-off particles by (map, neighbour slot).
 #include "pa_internal.hpp"
+#include "pa_color.hpp"
 
 #pragma clang fp contract(off)
 
@@ -219,6 +220,67 @@ __global__ __launch_bounds__(kBlock) void k_copy_counted(const float4* __restric
   }
 }
 
+// Counted prune, index generation and colouring of seed-major slots in one
+// pass (pipeline.cpp:186-260 runs prune, index_generator.cpp:10-24 and
+// color_generator.cpp:16-98 in turn over each chunk).  A wave per seed p
+// copies its c = count[p] slots to O = offs[p], 64 at a time, and checks
+// that the curve has the shape the tracers write: c >= 2, no termination
+// vertex before the last slot, a termination vertex last.  For such curves
+// the index generator's pairs are the c - 2 pairs of the non-termination
+// vertices, at index positions O - 2p onward, and the chunk has m - 2n
+// indices; the colours are k_color's: colors[k] = segment(v[k], v[k+1]) for
+// each index k, and the curve's last vertex takes its last segment's colour
+// unless that segment is the chunk's final index (then every later seed has
+// c == 2).  Indices and colours are written from the vertices in registers
+// (v[k+1] from the next lane) as the copy goes, before the whole curve has
+// been checked; a curve of any other shape sets *irregular, and the host
+// then redoes indices and colours with the general kernels, which overwrite
+// every position that they write.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_finish_counted(const float4* __restrict__ slots, int64_t stride,
+                                                           const int32_t* __restrict__ counts,
+                                                           const int64_t* __restrict__ offs, int64_t n,
+                                                           float4* __restrict__ out, int32_t* __restrict__ idx,
+                                                           float4* __restrict__ colors, float param,
+                                                           int32_t* __restrict__ irregular) {
+  const int64_t waves = int64_t(gridDim.x) * (kBlock / 64);
+  const int64_t n_idx = offs[n] - 2 * n;
+  const int lane = lane_id();
+  for (int64_t p = int64_t(blockIdx.x) * (kBlock / 64) + wave_id(); p < n; p += waves) {
+    const int32_t c = counts[p];
+    const int64_t o = offs[p];
+    const float4* src = slots + p * stride;
+    float4* dst = out + o;
+    const int64_t first = o - 2 * p;  // index position of the curve's first pair
+    const bool last_is_final = first + (c - 2) == n_idx;
+    bool bad = c < 2;
+    for (int j0 = 0; j0 < c; j0 += 64) {
+      const int j = j0 + lane;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (j < c) {
+        v = src[j];
+        dst[j] = v;
+        bad |= is_termination(v) != (j == c - 1);
+      }
+      float4 w;  // v[j + 1]
+      w.x = __shfl_down(v.x, 1);
+      w.y = __shfl_down(v.y, 1);
+      w.z = __shfl_down(v.z, 1);
+      w.w = __shfl_down(v.w, 1);
+      if (lane == 63 && j + 1 < c) w = src[j + 1];
+      if (j < c - 2) {
+        const float4 col = segment_color(v, w, MODE, param);
+        idx[first + j] = int32_t(o + j);
+        colors[o + j] = col;
+        if (j == c - 3 && !last_is_final) colors[o + j + 1] = col;
+      }
+    }
+    if (__ballot(bad) != 0 && lane == 0) atomicOr(irregular, 1);
+  }
+}
+
+__global__ void k_zero_flag(int32_t* f) { *f = 0; }
+
 // Step-major slots (slot s of seed p at s * n + p) into the pruned layout:
 // a workgroup takes 64 consecutive seeds and moves 64-slot tiles through LDS,
 // reading whole rows (1 KiB per wave) and writing each seed's run contiguously.
@@ -380,6 +442,36 @@ pa_status group_handoffs(pa_gpu_ctx* ctx, const int32_t* d_code, const pa_partic
 
 using namespace pa_amd;
 
+namespace {
+// The seed-major counted prunes' output offsets: offs[0..n] (d_offsets, or
+// scratch) = the exclusive scan of counts, *total = offs[n] (read back; more
+// than capacity is PA_E_CAPACITY).  extra > 0 bytes of zeroed-by-caller
+// scratch after the scan's workspace come back in *extra_out.
+pa_status counted_offsets(pa_gpu_ctx* ctx, const int32_t* d_counts, size_t n, size_t capacity, int64_t* d_offsets,
+                          size_t extra, int64_t** offs_out, int32_t** extra_out, size_t* total_out) {
+  int64_t* offs = d_offsets;
+  void* ws = nullptr;
+  const size_t own = d_offsets ? 0 : (n + 1) * sizeof(int64_t);
+  PA_TRY(scratch(ctx, own + scan_ws_total(int64_t(n)) + 4096 + extra, &ws));
+  if (!offs) offs = static_cast<int64_t*>(ws);
+  int64_t* sws = reinterpret_cast<int64_t*>(static_cast<char*>(ws) + own);
+  if (extra_out)
+    *extra_out = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + own + scan_ws_total(int64_t(n)) + 4096);
+  launch_begin(ctx, "prune_scan");
+  PA_TRY(scan_rec<int32_t>(ctx, d_counts, int64_t(n), offs, sws));
+  launch_end(ctx);
+  int64_t total = 0;
+  PA_TRY(capture_guard(ctx));
+  PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *total_out = size_t(total);
+  *offs_out = offs;
+  if (size_t(total) > capacity) return fail(ctx, PA_E_CAPACITY, "prune_counted: output capacity too small");
+  return PA_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 pa_status pa_gpu_prune(pa_gpu_ctx* ctx, const float* d_vertices, size_t n, float* d_out, size_t* n_out) {
@@ -414,21 +506,8 @@ pa_status pa_gpu_prune_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t str
     *n_out = 0;
     return PA_OK;
   }
-  int64_t* offs = d_offsets;
-  void* ws = nullptr;
-  const size_t own = d_offsets ? 0 : (n + 1) * sizeof(int64_t);
-  PA_TRY(scratch(ctx, own + scan_ws_total(int64_t(n)) + 4096, &ws));
-  if (!offs) offs = static_cast<int64_t*>(ws);
-  int64_t* sws = reinterpret_cast<int64_t*>(static_cast<char*>(ws) + own);
-  launch_begin(ctx, "prune_scan");
-  PA_TRY(scan_rec<int32_t>(ctx, d_vertex_counts, int64_t(n), offs, sws));
-  launch_end(ctx);
-  int64_t total = 0;
-  PA_TRY(capture_guard(ctx));
-  PA_HIP(ctx, hipMemcpyAsync(&total, offs + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
-  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *n_out = size_t(total);
-  if (size_t(total) > capacity) return fail(ctx, PA_E_CAPACITY, "prune_counted: output capacity too small");
+  int64_t* offs = nullptr;
+  PA_TRY(counted_offsets(ctx, d_vertex_counts, n, capacity, d_offsets, 0, &offs, nullptr, n_out));
   int64_t blocks = (int64_t(n) + 3) / 4;
   if (blocks > 65536) blocks = 65536;
   launch_begin(ctx, "prune_counted");
@@ -438,6 +517,50 @@ pa_status pa_gpu_prune_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t str
   launch_end(ctx);
   PA_HIP(ctx, hipGetLastError());
   return PA_OK;
+}
+
+pa_status pa_gpu_finish_counted(pa_gpu_ctx* ctx, const float* d_slots, size_t stride_slots,
+                                const int32_t* d_vertex_counts, size_t n, float* d_vertices, size_t capacity,
+                                size_t* n_vertices, int64_t* d_offsets, int32_t* d_indices, size_t* n_indices,
+                                pa_color_mode mode, float parameter, float* d_colors) {
+  if (!ctx || !n_vertices || !n_indices) return PA_E_INVALID;
+  if (n && (!d_slots || !d_vertex_counts || !d_vertices || !d_indices || !d_colors)) return PA_E_INVALID;
+  if (!with_color_mode(int(mode), [](auto) {})) return fail(ctx, PA_E_INVALID, "unknown color mode");
+  *n_vertices = 0;
+  *n_indices = 0;
+  if (n == 0) return PA_OK;
+  if (capacity > size_t(INT32_MAX) + 1) capacity = size_t(INT32_MAX) + 1;  // int32 indices address the chunk
+  int64_t* offs = nullptr;
+  int32_t* flag = nullptr;
+  size_t m = 0;
+  PA_TRY(counted_offsets(ctx, d_vertex_counts, n, capacity, d_offsets, 256, &offs, &flag, &m));
+  *n_vertices = m;
+  hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ctx->stream, flag);
+  int64_t blocks = (int64_t(n) + 3) / 4;
+  if (blocks > 65536) blocks = 65536;
+  launch_begin(ctx, "finish_counted");
+  with_color_mode(int(mode), [&](auto M) {
+    hipLaunchKernelGGL(k_finish_counted<decltype(M)::value>, dim3(unsigned(blocks)), dim3(kBlock), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(d_slots), int64_t(stride_slots), d_vertex_counts, offs,
+                       int64_t(n), reinterpret_cast<float4*>(d_vertices), d_indices,
+                       reinterpret_cast<float4*>(d_colors), parameter, flag);
+  });
+  launch_end(ctx);
+  PA_HIP(ctx, hipGetLastError());
+  int32_t irregular = 0;
+  PA_HIP(ctx, hipMemcpyAsync(&irregular, flag, sizeof irregular, hipMemcpyDeviceToHost, ctx->stream));
+  PA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (!irregular) {
+    *n_indices = m - 2 * n;
+    return PA_OK;
+  }
+  // a curve of another shape: the general index generator and colouring over the pruned array
+  if (m < 2) return PA_OK;
+  size_t ni = 0;
+  PA_TRY(pa_gpu_index_generate(ctx, d_vertices, m, d_indices, &ni));
+  *n_indices = ni;
+  return color_launch(ctx, reinterpret_cast<const float4*>(d_vertices), d_indices, int64_t(ni), int(mode),
+                      parameter, reinterpret_cast<float4*>(d_colors));
 }
 
 pa_status pa_gpu_prune_counted_chunked(pa_gpu_ctx* ctx, const float* d_slots, size_t region_slots,
