@@ -445,8 +445,8 @@ using namespace pa_amd;
 namespace {
 // The seed-major counted prunes' output offsets: offs[0..n] (d_offsets, or
 // scratch) = the exclusive scan of counts, *total = offs[n] (read back; more
-// than capacity is PA_E_CAPACITY).  extra > 0 bytes of zeroed-by-caller
-// scratch after the scan's workspace come back in *extra_out.
+// than capacity is PA_E_CAPACITY).  `extra` more bytes of scratch, after the
+// scan's workspace, come back in *extra_out for the caller's own use.
 pa_status counted_offsets(pa_gpu_ctx* ctx, const int32_t* d_counts, size_t n, size_t capacity, int64_t* d_offsets,
                           size_t extra, int64_t** offs_out, int32_t** extra_out, size_t* total_out) {
   int64_t* offs = d_offsets;
